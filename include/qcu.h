@@ -1,0 +1,136 @@
+/*
+ * qcu.h -- C ABI of the MI355X-native qcu hot path (libqcu.so).
+ *
+ * The first block keeps the six entry points of the reference's include/qcu.h
+ * (include/qcu.h:17-30) with identical signatures and meanings, so that the
+ * PyQcu Cython binding (python/PyQuda/pyquda/src/qcu.pxd:1-8) and any C caller
+ * link against this library unchanged:
+ *   - every pointer is a DEVICE pointer in the external PyQuda layouts
+ *       gauge   complex128 [4 mu][2 parity][Lt][Lz][Ly][Lx/2][3][3]
+ *       fermion complex128 [Lt][Lz][Ly][Lx/2][4 spin][3 colour]   (one parity)
+ *       full    complex128 [2 parity][Lt][Lz][Ly][Lx/2][4][3]     (even first)
+ *   - parity is the OUTPUT parity; the input fermion lives on the other one;
+ *   - boundaries are periodic in all four directions (the reference kernel
+ *     wraps periodically, include/define.h:127-134);
+ *   - calls are synchronous on return and leave every input bit-identical
+ *     (the reference transposes gauge and fermion_in in place and back).
+ * Errors never abort the process and never throw: the void entry points record
+ * the failure, readable with qcuGetLastError()/qcuGetLastErrorString().
+ *
+ * The second block is new: handle-based entry points with runtime kappa,
+ * c_sw, time boundary, tolerance, precision, batching and the learned
+ * preconditioner.  They return 0 on success or an error code.
+ */
+#ifndef QCU_H
+#define QCU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct QcuParam_s {
+  int lattice_size[4];
+} QcuParam;
+
+/* ---- reference entry points (include/qcu.h:21-30) ------------------------ */
+
+/* out = D_{p,1-p} in: Wilson hop on output parity p, no kappa (src/draft.cu:3-28) */
+void dslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity);
+/* out = A_p^-1 D_{p,1-p} in with A = 1 - sum sigma F at kappa*csw = 1
+ * (src/nccl_clover_dslash.cu:3-79) */
+void dslashCloverQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity);
+/* empty in the reference (src/draft.cu:29); here the same operator as
+ * ncclDslashQcu, grid must be {1,1,1,1} */
+void mpiDslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity,
+                  QcuParam *grid);
+/* empty in the reference (src/draft.cu:30-31); kept as a no-op */
+void mpiBistabCgQcu(void *gauge, QcuParam *param, QcuParam *grid);
+/* Wilson hop, one GPU: grid must be {1,1,1,1} (src/nccl_wilson_dslash.cu:3-19) */
+void ncclDslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity,
+                   QcuParam *grid);
+/* solves (1 - kappa D) x = b on the full lattice, kappa = 0.125, even-odd
+ * BiCGSTAB on the odd sites stopped when Re<r,r> < 1e-9 or after 1000
+ * iterations (src/nccl_wilson_bistabcg.cu:7-27, include/define.h:67-69);
+ * fermion_out = x, fermion_in = b, full-lattice layout */
+void ncclBistabCgQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param,
+                     QcuParam *grid);
+
+/* ---- errors (new) ------------------------------------------------------- */
+int qcuGetLastError(void);
+const char *qcuGetLastErrorString(void);
+void qcuClearError(void);
+
+/* ---- handle-based API (new) --------------------------------------------- */
+typedef struct QcuLattice_s *QcuLatticeHandle;
+
+enum {
+  QCU_PREC_FP64 = 0,
+  QCU_PREC_FP32 = 1,
+  QCU_PREC_FP16 = 2 /* half storage with one FP32 scale per site */
+};
+
+/* kappa, csw (0 => Wilson), t_boundary +1 periodic / -1 antiperiodic */
+int qcuLatticeCreate(QcuLatticeHandle *h, const int lattice_size[4], double kappa, double csw,
+                     int t_boundary);
+int qcuLatticeDestroy(QcuLatticeHandle h);
+/* gauge: device pointer, external layout; builds the resident links (FP64,
+ * FP32, FP16) and, for csw != 0, the packed clover term and its inverse */
+int qcuLatticeLoadGauge(QcuLatticeHandle h, const void *gauge);
+/* fills a device gauge buffer (external layout) with U = exp(i eps H), seeded */
+int qcuGaugeRandom(void *gauge, const int lattice_size[4], double eps, uint64_t seed);
+/* fills n complex128 values with Gaussian (re, im ~ N(0,1)) numbers, seeded */
+int qcuGaussian(void *field, long n_complex, uint64_t seed);
+
+/* out = [A_p^-1] D_{p,1-p} in on external one-parity fields; clover = 1 adds
+ * the inverse clover of the lattice; prec selects the kernel precision */
+int qcuDslash(QcuLatticeHandle h, void *out, const void *in, int parity, int clover, int prec);
+/* out = M in = (A - kappa D) in on the full lattice, FP64 */
+int qcuMatFull(QcuLatticeHandle h, void *out, const void *in);
+/* |b - M x| / |b| in FP64 on the full lattice */
+int qcuTrueResidual(QcuLatticeHandle h, const void *x, const void *b, double *res);
+
+typedef struct QcuSolveParam_s {
+  double tol;            /* target |b - M x| / |b| (full lattice, FP64) */
+  int max_iter;          /* BiCGSTAB iterations (sloppy iterations count) */
+  int precision;         /* QCU_PREC_*: inner (sloppy) precision */
+  double reliable_delta; /* reliable update when |r| < delta |r_last_update| */
+  int chunk;             /* iterations between host checks (graph length) */
+  int use_graph;         /* capture the chunk as a hipGraph and replay it */
+  void *precond;         /* QcuNetHandle or NULL */
+  int legacy_stop;       /* 1: stop on Re<r,r> < tol absolute (reference rule) */
+} QcuSolveParam;
+
+typedef struct QcuSolveInfo_s {
+  int iters;
+  int reliable_updates;
+  int converged;
+  int fallback;          /* 1: the preconditioned solve stalled, plain solve used */
+  double true_res;
+  double seconds;
+  int restarts;
+  int pad;
+} QcuSolveInfo;
+
+/* x, b: full-lattice device fields in the external layout */
+int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p, QcuSolveInfo *info);
+
+/* Times `iters` launches of the internal hop kernel on resident fields
+ * (xpay = 0: out = [A^-1] D in; 1: the second half of the Schur operator) with
+ * HIP events on the library's stream; block / remap override the tuned launch
+ * shape when > 0 / >= 0.  ms = average milliseconds per launch. */
+int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap,
+                double *ms);
+
+/* launch-shape autotuner: sweeps block size and XCD-aware block order of the
+ * hop kernel for this lattice and precision and keeps the fastest; the table
+ * persists in a text file */
+int qcuTune(QcuLatticeHandle h, int prec, double *best_ms);
+int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap);
+int qcuTuneSave(const char *path);
+int qcuTuneLoad(const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

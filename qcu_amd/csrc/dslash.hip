@@ -1,0 +1,270 @@
+// dslash.hip -- even-odd Wilson / clover hopping term for gfx950.
+//
+// out_p(x) = sum_mu (1 - g_mu) U_mu(x) psi(x+mu) + (1 + g_mu) U_mu(x-mu)^dag psi(x-mu)
+// on the sites of output parity p, psi on parity 1-p: the operator of
+// src/wilson_dslash.cu:9-245 with the spin projectors of the DeGrand-Rossi
+// basis written out as half-spinor (two 3-vectors) products, the 12-real link
+// reconstruction of include/define.h:198-206, periodic wrap in every direction
+// (the single-GPU case of include/lattice_dslash.h:11-197) and an optional -1
+// phase on the hops that cross the time boundary.
+//
+// One thread per output site, site-fastest SoA loads (coalesced 16-byte lanes),
+// eight directions fused, optional epilogues: packed clover inverse (two
+// Hermitian 6x6 blocks, src/clover_dslash.cu:857-882 restricted to its
+// non-zero chiral blocks), x - k2 * (...) for the even-odd Schur operator
+// (src/bistabcg.cu:85-96), and fused FP64 grid reductions feeding the BiCGSTAB
+// scalar epilogues.  The stencil is HBM bound (about 0.5 flop per byte), so
+// no matrix cores are involved.
+#include "qcu_internal.h"
+#include "qcu_reduce.h"
+#include "qcu_state.h"
+
+namespace qcu {
+
+template <class T>
+struct Half {
+  cplx<T> h0[3], h1[3];
+};
+
+// spin projection of psi(neighbour) for direction MU, FWD = 1: (1 - g_mu), 0: (1 + g_mu)
+template <int MU, int FWD, class T>
+QCU_D void project(const cplx<T> (&s)[12], cplx<T> (&h0)[3], cplx<T> (&h1)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const cplx<T> s0 = s[c], s1 = s[3 + c], s2 = s[6 + c], s3 = s[9 + c];
+    if (MU == 0) {
+      h0[c] = FWD ? s0 + mul_mi(s3) : s0 + mul_i(s3);
+      h1[c] = FWD ? s1 + mul_mi(s2) : s1 + mul_i(s2);
+    } else if (MU == 1) {
+      h0[c] = FWD ? s0 + s3 : s0 - s3;
+      h1[c] = FWD ? s1 - s2 : s1 + s2;
+    } else if (MU == 2) {
+      h0[c] = FWD ? s0 + mul_mi(s2) : s0 + mul_i(s2);
+      h1[c] = FWD ? s1 + mul_i(s3) : s1 + mul_mi(s3);
+    } else {
+      h0[c] = FWD ? s0 - s2 : s0 + s2;
+      h1[c] = FWD ? s1 - s3 : s1 + s3;
+    }
+  }
+}
+
+// add the reconstructed 4-spinor of the transported half spinor (a0, a1)
+template <int MU, int FWD, class T>
+QCU_D void accumulate(cplx<T> (&d)[12], const cplx<T> (&a0)[3], const cplx<T> (&a1)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    d[c] = d[c] + a0[c];
+    d[3 + c] = d[3 + c] + a1[c];
+    if (MU == 0) {
+      d[6 + c] = FWD ? d[6 + c] + mul_i(a1[c]) : d[6 + c] + mul_mi(a1[c]);
+      d[9 + c] = FWD ? d[9 + c] + mul_i(a0[c]) : d[9 + c] + mul_mi(a0[c]);
+    } else if (MU == 1) {
+      d[6 + c] = FWD ? d[6 + c] - a1[c] : d[6 + c] + a1[c];
+      d[9 + c] = FWD ? d[9 + c] + a0[c] : d[9 + c] - a0[c];
+    } else if (MU == 2) {
+      d[6 + c] = FWD ? d[6 + c] + mul_i(a0[c]) : d[6 + c] + mul_mi(a0[c]);
+      d[9 + c] = FWD ? d[9 + c] + mul_mi(a1[c]) : d[9 + c] + mul_i(a1[c]);
+    } else {
+      d[6 + c] = FWD ? d[6 + c] - a0[c] : d[6 + c] + a0[c];
+      d[9 + c] = FWD ? d[9 + c] - a1[c] : d[9 + c] + a1[c];
+    }
+  }
+}
+
+// a = U h (FWD) or U^dag h
+template <int FWD, class T>
+QCU_D void transport(const cplx<T> (&u)[9], const cplx<T> (&h)[3], cplx<T> (&a)[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    cplx<T> acc = mk<T>(0, 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (FWD)
+        cmad(acc, u[r * 3 + c], h[c]);
+      else
+        cmadj(acc, u[c * 3 + r], h[c]);
+    }
+    a[r] = acc;
+  }
+}
+
+template <int P, int MU, int FWD>
+QCU_D void hop_dir(cplx<typename real_of<P>::type> (&d)[12], const LinkPtr &U, int pm, int li,
+                   const SpinorPtr &in, int si, typename real_of<P>::type ph) {
+  typedef typename real_of<P>::type T;
+  cplx<T> s[12];
+  SpinorIO<P>::load(in, si, s);
+  cplx<T> h0[3], h1[3];
+  project<MU, FWD>(s, h0, h1);
+  cplx<T> u[9];
+  LinkIO<P>::load(U, pm, li, u);
+  cplx<T> a0[3], a1[3];
+  transport<FWD>(u, h0, a0);
+  transport<FWD>(u, h1, a1);
+  if (MU == 3) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a0[c] = ph * a0[c];
+      a1[c] = ph * a1[c];
+    }
+  }
+  accumulate<MU, FWD>(d, a0, a1);
+}
+
+struct HopKArgs {
+  SpinorPtr out, in, x, y;
+  LinkPtr U;
+  CloverPtr C;
+  Geom g;
+  int parity;
+  int remap;
+  double k2;
+  SysState *st;
+  double *partials;
+  unsigned *counter;
+};
+
+template <int P, int CLOVER, int EPI, int RED>
+__global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
+  typedef typename real_of<P>::type T;
+  // a solver state, when given, stops the chain of kernels once the solve has stopped
+  if (a.st && a.st->done) return;
+  const int blk = remap_block(blockIdx.x, gridDim.x, a.remap);
+  const int i = blk * blockDim.x + threadIdx.x;
+  const Geom &g = a.g;
+  const bool active = i < g.vh;
+  cplx<T> d[12];
+  double red[3] = {0, 0, 0};
+  if (active) {
+    const Site s = decode(g, i, a.parity);
+    const int p = a.parity, q = 1 - p;
+    const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) d[k] = mk<T>(0, 0);
+    // x: a site with odd full x has its x-1 neighbour at the same half index
+    {
+      int jb = s.xodd ? i : (s.xh == 0 ? i + g.lxh - 1 : i - 1);
+      int jf = s.xodd ? (s.xh == g.lxh - 1 ? i - g.lxh + 1 : i + 1) : i;
+      hop_dir<P, 0, 0>(d, a.U, q * 4 + 0, jb, a.in, jb, (T)1);
+      hop_dir<P, 0, 1>(d, a.U, p * 4 + 0, i, a.in, jf, (T)1);
+    }
+    {
+      int jb = s.y == 0 ? i + (g.ly - 1) * sy : i - sy;
+      int jf = s.y == g.ly - 1 ? i - (g.ly - 1) * sy : i + sy;
+      hop_dir<P, 1, 0>(d, a.U, q * 4 + 1, jb, a.in, jb, (T)1);
+      hop_dir<P, 1, 1>(d, a.U, p * 4 + 1, i, a.in, jf, (T)1);
+    }
+    {
+      int jb = s.z == 0 ? i + (g.lz - 1) * sz : i - sz;
+      int jf = s.z == g.lz - 1 ? i - (g.lz - 1) * sz : i + sz;
+      hop_dir<P, 2, 0>(d, a.U, q * 4 + 2, jb, a.in, jb, (T)1);
+      hop_dir<P, 2, 1>(d, a.U, p * 4 + 2, i, a.in, jf, (T)1);
+    }
+    {
+      int jb = s.t == 0 ? i + (g.lt - 1) * st_ : i - st_;
+      int jf = s.t == g.lt - 1 ? i - (g.lt - 1) * st_ : i + st_;
+      T phb = (s.t == 0) ? (T)g.tbc : (T)1;
+      T phf = (s.t == g.lt - 1) ? (T)g.tbc : (T)1;
+      hop_dir<P, 3, 0>(d, a.U, q * 4 + 3, jb, a.in, jb, phb);
+      hop_dir<P, 3, 1>(d, a.U, p * 4 + 3, i, a.in, jf, phf);
+    }
+    if (CLOVER) {
+      T c[72];
+      CloverIO<P>::load(a.C, i, c);
+      clover_apply_site(c, d);
+    }
+    if (EPI == EPI_XPAY) {
+      cplx<T> x[12];
+      SpinorIO<P>::load(a.x, i, x);
+      const T k2 = (T)a.k2;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) d[k] = x[k] - k2 * d[k];
+      if (RED == RED_OUT_X) {
+        // <out, x> and |out|^2
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          red[0] += (double)d[k].re * (double)x[k].re + (double)d[k].im * (double)x[k].im;
+          red[1] += (double)d[k].re * (double)x[k].im - (double)d[k].im * (double)x[k].re;
+          red[2] += (double)d[k].re * (double)d[k].re + (double)d[k].im * (double)d[k].im;
+        }
+      }
+    }
+    if (RED == RED_DOT_Y_OUT) {
+      cplx<T> y[12];
+      SpinorIO<P>::load(a.y, i, y);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        red[0] += (double)y[k].re * (double)d[k].re + (double)y[k].im * (double)d[k].im;
+        red[1] += (double)y[k].re * (double)d[k].im - (double)y[k].im * (double)d[k].re;
+      }
+    }
+    SpinorIO<P>::store(a.out, i, d);
+  }
+  if (RED == RED_DOT_Y_OUT) {
+    double v[2] = {red[0], red[1]}, tot[2];
+    if (grid_reduce<2>(v, a.partials, a.counter, blockIdx.x, gridDim.x, tot)) fin_alpha(a.st, tot[0], tot[1]);
+  } else if (RED == RED_OUT_X) {
+    double tot[3];
+    if (grid_reduce<3>(red, a.partials, a.counter, blockIdx.x, gridDim.x, tot))
+      fin_omega(a.st, tot[0], tot[1], tot[2]);
+  }
+}
+
+template <int P, int CLOVER, int EPI, int RED>
+static int launch(const HopKArgs &a, const LaunchCfg &cfg, hipStream_t s) {
+  int nb = (a.g.vh + cfg.block - 1) / cfg.block;
+  hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED>), dim3(nb), dim3(cfg.block), 0, s, a);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int P>
+static int dispatch(const HopKArgs &a, int clover, int epi, int red, const LaunchCfg &cfg,
+                    hipStream_t s) {
+  if (epi == EPI_PLAIN) {
+    return clover ? launch<P, 1, EPI_PLAIN, RED_NONE>(a, cfg, s)
+                  : launch<P, 0, EPI_PLAIN, RED_NONE>(a, cfg, s);
+  }
+  if (red == RED_NONE)
+    return clover ? launch<P, 1, EPI_XPAY, RED_NONE>(a, cfg, s)
+                  : launch<P, 0, EPI_XPAY, RED_NONE>(a, cfg, s);
+  if (red == RED_DOT_Y_OUT)
+    return clover ? launch<P, 1, EPI_XPAY, RED_DOT_Y_OUT>(a, cfg, s)
+                  : launch<P, 0, EPI_XPAY, RED_DOT_Y_OUT>(a, cfg, s);
+  return clover ? launch<P, 1, EPI_XPAY, RED_OUT_X>(a, cfg, s)
+                : launch<P, 0, EPI_XPAY, RED_OUT_X>(a, cfg, s);
+}
+
+int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, SpinorPtr x, double k2,
+        int red, SpinorPtr y, SysState *st, double *partials, unsigned *counter, const LaunchCfg &cfg,
+        hipStream_t s, bool use_clover) {
+  QCU_CHECK(parity == 0 || parity == 1, QCU_ERR_ARG, "parity must be 0 or 1");
+  QCU_CHECK(cfg.block >= 64 && cfg.block <= 1024 && cfg.block % 64 == 0, QCU_ERR_ARG,
+            "block size must be a multiple of 64 in [64, 1024]");
+  QCU_CHECK(red == RED_NONE || (epi == EPI_XPAY && st && partials && counter), QCU_ERR_ARG,
+            "fused reductions need the xpay epilogue and solver state");
+  HopKArgs a;
+  a.out = out;
+  a.in = in;
+  a.x = x;
+  a.y = y;
+  a.U = op.U;
+  a.C = op.C[parity];
+  a.g = op.g;
+  a.parity = parity;
+  a.remap = cfg.remap;
+  a.k2 = k2;
+  a.st = st;
+  a.partials = partials;
+  a.counter = counter;
+  int clover = use_clover && op.C[parity].p != nullptr;
+  switch (op.prec) {
+    case P64: return dispatch<P64>(a, clover, epi, red, cfg, s);
+    case P32: return dispatch<P32>(a, clover, epi, red, cfg, s);
+    case P16: return dispatch<P16>(a, clover, epi, red, cfg, s);
+  }
+  set_error(QCU_ERR_ARG, "unknown precision");
+  return QCU_ERR_ARG;
+}
+
+}  // namespace qcu

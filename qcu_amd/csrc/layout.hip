@@ -1,0 +1,111 @@
+// layout.hip -- external (PyQuda) <-> internal field layouts.
+//
+// External layouts (the C ABI, python/PyQuda/pyquda/field.py:72-78):
+//   gauge  complex128 [4 mu][2 p][Vh][3][3]
+//   spinor complex128 [Vh][4][3]  (one parity)
+// The reference converts in place through a temporary and cuBLAS copies
+// (src/lattice_cuda.cu:41-211); here the caller's buffers are only read (or
+// only written), and the internal copies stay resident.
+#include "qcu_internal.h"
+
+namespace qcu {
+
+// one thread per (p, mu, site): rows 0 and 1 of the 3x3 link
+template <int P>
+__global__ void k_gauge_in(const double2 *ext, Geom g, void *a, void *b) {
+  const int vh = g.vh;
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= 8L * vh) return;
+  const int pm = (int)(n / vh), i = (int)(n - (long)pm * vh);
+  const int mu = pm >> 1, p = pm & 1;  // external order [mu][p]
+  const double2 *src = ext + n * 9;
+  double2 u[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) u[k] = src[k];
+  const int slab = p * 4 + mu;  // internal order [p][mu]
+  if (P == P64) {
+    double2 *dst = (double2 *)a + (size_t)slab * 6 * vh + i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[(size_t)k * vh] = u[k];
+  } else if (P == P32) {
+    float4 *dst = (float4 *)a + (size_t)slab * 3 * vh + i;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      dst[(size_t)k * vh] = make_float4((float)u[2 * k].x, (float)u[2 * k].y, (float)u[2 * k + 1].x,
+                                        (float)u[2 * k + 1].y);
+  } else {
+    unsigned w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      w[k] = (unsigned)f2h((float)u[k].x) | ((unsigned)f2h((float)u[k].y) << 16);
+    ((uint4 *)a)[(size_t)slab * vh + i] = make_uint4(w[0], w[1], w[2], w[3]);
+    ((uint2 *)b)[(size_t)slab * vh + i] = make_uint2(w[4], w[5]);
+  }
+}
+
+int gauge_ext_to_internal(const void *ext, const Geom &g, int prec, void *a, void *b, hipStream_t s) {
+  const long n = 8L * g.vh;
+  const int bs = 256;
+  const int nb = (int)((n + bs - 1) / bs);
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_gauge_in<P64>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g, a, b); break;
+    case P32: hipLaunchKernelGGL(k_gauge_in<P32>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g, a, b); break;
+    case P16: hipLaunchKernelGGL(k_gauge_in<P16>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g, a, b); break;
+    default: set_error(QCU_ERR_ARG, "bad precision"); return QCU_ERR_ARG;
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int P>
+__global__ void k_spinor_in(const double2 *ext, int vh, SpinorPtr dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  typedef typename real_of<P>::type T;
+  cplx<T> v[12];
+  const double2 *src = ext + (size_t)i * 12;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    double2 a = src[k];
+    v[k] = mk<T>((T)a.x, (T)a.y);
+  }
+  SpinorIO<P>::store(dst, i, v);
+}
+
+template <int P>
+__global__ void k_spinor_out(SpinorPtr src, int vh, double2 *ext) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  typedef typename real_of<P>::type T;
+  cplx<T> v[12];
+  SpinorIO<P>::load(src, i, v);
+  double2 *dst = ext + (size_t)i * 12;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) dst[k] = make_double2((double)v[k].re, (double)v[k].im);
+}
+
+int spinor_ext_to_internal(const void *ext, const Geom &g, SpinorPtr dst, int prec, hipStream_t s) {
+  const int bs = 256, nb = (g.vh + bs - 1) / bs;
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_spinor_in<P64>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
+    case P32: hipLaunchKernelGGL(k_spinor_in<P32>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
+    case P16: hipLaunchKernelGGL(k_spinor_in<P16>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
+    default: set_error(QCU_ERR_ARG, "bad precision"); return QCU_ERR_ARG;
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int spinor_internal_to_ext(SpinorPtr src, const Geom &g, void *ext, int prec, hipStream_t s) {
+  const int bs = 256, nb = (g.vh + bs - 1) / bs;
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_spinor_out<P64>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
+    case P32: hipLaunchKernelGGL(k_spinor_out<P32>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
+    case P16: hipLaunchKernelGGL(k_spinor_out<P16>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
+    default: set_error(QCU_ERR_ARG, "bad precision"); return QCU_ERR_ARG;
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcu

@@ -1,0 +1,161 @@
+"""Handle-based host interface over libqcu.so for torch device tensors.
+
+Fields use the external (PyQuda) layouts of qcu's C ABI:
+  gauge   complex128 [4][2][Lt][Lz][Ly][Lx/2][3][3]
+  half    complex128 [Lt][Lz][Ly][Lx/2][4][3]   (one parity)
+  full    complex128 [2][Lt][Lz][Ly][Lx/2][4][3]
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import SolveInfo, SolveParam
+
+FP64, FP32, FP16 = 0, 1, 2
+PREC = {"fp64": FP64, "fp32": FP32, "fp16": FP16}
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError("expected a GPU torch tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def vol(L):
+    return int(L[0] * L[1] * L[2] * L[3])
+
+
+def gauge_shape(L):
+    Lx, Ly, Lz, Lt = L
+    return (4, 2, Lt, Lz, Ly, Lx // 2, 3, 3)
+
+
+def fermion_shape(L, full=True):
+    Lx, Ly, Lz, Lt = L
+    return ((2,) if full else ()) + (Lt, Lz, Ly, Lx // 2, 4, 3)
+
+
+def random_gauge(L, eps, seed, device="cuda"):
+    """U = exp(i eps H), H = sum_a w_a lambda_a with w_a ~ N(0,1), generated on the GPU."""
+    U = torch.empty(gauge_shape(L), dtype=torch.complex128, device=device)
+    _lib.check(_lib.load().qcuGaugeRandom(_ptr(U), (C.c_int * 4)(*L), float(eps), int(seed)))
+    return U
+
+
+def gaussian(shape, seed, device="cuda"):
+    """complex128 field with real and imaginary parts ~ N(0,1), generated on the GPU."""
+    v = torch.empty(shape, dtype=torch.complex128, device=device)
+    _lib.check(_lib.load().qcuGaussian(_ptr(v), v.numel(), int(seed)))
+    return v
+
+
+def point_source(L, spin=0, colour=0, device="cuda"):
+    """1 at site (0,0,0,0), as python/PyQuda/pyquda/utils/source.py:7-22"""
+    b = torch.zeros(fermion_shape(L), dtype=torch.complex128, device=device)
+    b[0, 0, 0, 0, 0, spin, colour] = 1
+    return b
+
+
+@dataclass
+class SolveResult:
+    iters: int
+    reliable_updates: int
+    converged: bool
+    fallback: bool
+    true_res: float
+    seconds: float
+    restarts: int
+
+
+class Lattice:
+    """Resident gauge field (+ clover) for one (L, kappa, csw, t_boundary)."""
+
+    def __init__(self, L, kappa, csw=0.0, t_boundary=-1):
+        self.L = tuple(int(v) for v in L)
+        self.kappa = float(kappa)
+        self.csw = float(csw)
+        self.t_boundary = int(t_boundary)
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.qcuLatticeCreate(C.byref(h), (C.c_int * 4)(*self.L), self.kappa,
+                                              self.csw, self.t_boundary))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.qcuLatticeDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def vh(self):
+        return vol(self.L) // 2
+
+    def load_gauge(self, U: torch.Tensor):
+        assert U.dtype == torch.complex128 and U.numel() == 36 * vol(self.L)
+        _lib.check(self._lib.qcuLatticeLoadGauge(self.h, _ptr(U)))
+        torch.cuda.synchronize()
+
+    def dslash(self, inp: torch.Tensor, parity: int, clover=False, prec=FP64) -> torch.Tensor:
+        out = torch.empty_like(inp)
+        _lib.check(self._lib.qcuDslash(self.h, _ptr(out), _ptr(inp), int(parity), int(clover),
+                                       int(prec)))
+        return out
+
+    def mat(self, x: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(x)
+        _lib.check(self._lib.qcuMatFull(self.h, _ptr(out), _ptr(x)))
+        return out
+
+    def true_residual(self, x, b) -> float:
+        r = C.c_double()
+        _lib.check(self._lib.qcuTrueResidual(self.h, _ptr(x), _ptr(b), C.byref(r)))
+        return r.value
+
+    def solve(self, b, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
+              use_graph=True, precond=None, legacy_stop=False, x=None):
+        if x is None:
+            x = torch.empty_like(b)
+        p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec),
+                       reliable_delta=reliable_delta, chunk=chunk, use_graph=int(use_graph),
+                       precond=None if precond is None else precond.handle,
+                       legacy_stop=int(legacy_stop))
+        info = SolveInfo()
+        _lib.check(self._lib.qcuSolve(self.h, _ptr(x), _ptr(b), C.byref(p), C.byref(info)))
+        return x, SolveResult(info.iters, info.reliable_updates, bool(info.converged),
+                              bool(info.fallback), info.true_res, info.seconds, info.restarts)
+
+    def bench_hop(self, prec=FP64, clover=None, xpay=True, iters=50, block=0, remap=-1):
+        """average ms per launch of the resident hop kernel, timed with HIP events"""
+        if clover is None:
+            clover = self.csw != 0.0
+        ms = C.c_double()
+        _lib.check(self._lib.qcuBenchHop(self.h, int(prec), int(bool(clover)), int(xpay),
+                                         int(iters), int(block), int(remap), C.byref(ms)))
+        return ms.value
+
+    def tune(self, prec=FP64):
+        ms = C.c_double()
+        _lib.check(self._lib.qcuTune(self.h, int(prec), C.byref(ms)))
+        b, r = C.c_int(), C.c_int()
+        _lib.check(self._lib.qcuTuneGet(self.h, int(prec), C.byref(b), C.byref(r)))
+        return {"block": b.value, "remap": r.value, "ms": ms.value}
+
+
+def tune_save(path):
+    _lib.check(_lib.load().qcuTuneSave(str(path).encode()))
+
+
+def tune_load(path):
+    _lib.check(_lib.load().qcuTuneLoad(str(path).encode()))

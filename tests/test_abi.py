@@ -1,0 +1,83 @@
+"""CPU checks of the C ABI library: it loads and exports every declared symbol.
+
+No compute call is made (there is no GPU in the build container); the
+reference-named entry points must match include/qcu.h:21-30 of the reference.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def qlib():
+    from qcu_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from qcu_amd.build import build
+        build()
+    return _lib
+
+
+def test_library_loads_and_exports_header(qlib):
+    lib = qlib.load()
+    syms = qlib.header_symbols()
+    for s in qlib.REFERENCE_SYMBOLS:
+        assert s in syms
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing
+    # every exported function is declared in the header with a ctypes signature
+    assert set(syms) == set(qlib.SIGNATURES)
+
+
+def test_exported_symbols_are_c_linkage(qlib):
+    out = subprocess.run(["nm", "-D", "--defined-only", qlib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for s in qlib.header_symbols():
+        assert s in names, s
+
+
+def test_header_matches_reference_signatures():
+    """the six reference prototypes, token for token (include/qcu.h:21-30)"""
+    import re
+    txt = open(os.path.join(ROOT, "include", "qcu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    flat = " ".join(txt.split())
+    expect = [
+        "void dslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity);",
+        "void dslashCloverQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity);",
+        "void mpiDslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity, QcuParam *grid);",
+        "void mpiBistabCgQcu(void *gauge, QcuParam *param, QcuParam *grid);",
+        "void ncclDslashQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, int parity, QcuParam *grid);",
+        "void ncclBistabCgQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam *param, QcuParam *grid);",
+        "typedef struct QcuParam_s { int lattice_size[4]; } QcuParam;",
+    ]
+    for e in expect:
+        assert e in flat, e
+
+
+def test_pyqcu_names_and_param():
+    from qcu_amd import pyqcu
+    for name in ("QcuParam", "dslashQcu", "dslashCloverQcu", "mpiDslashQcu", "mpiBistabCgQcu",
+                 "ncclDslashQcu", "ncclBistabCgQcu"):
+        assert hasattr(pyqcu, name)
+    p = pyqcu.QcuParam()
+    p.lattice_size = [8, 4, 6, 10]
+    assert p.lattice_size == [8, 4, 6, 10]
+    assert C.sizeof(p.param) == 16
+    with pytest.raises(ValueError):
+        p.lattice_size = [1, 2, 3]
+
+
+def test_pointer_shim():
+    import torch
+    from qcu_amd.pointer import Pointer, address, ndarrayDataPointer
+    t = torch.zeros(4, dtype=torch.complex128)
+    p = ndarrayDataPointer(t)
+    assert p.ptr == t.data_ptr() and p.dtype == "<c16"
+    assert address(Pointer("void", 1234)) == 1234
+    with pytest.raises(ValueError):
+        ndarrayDataPointer([1, 2, 3])
