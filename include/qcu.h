@@ -122,11 +122,11 @@ int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p,
 int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap,
                 double *ms);
 
-/* launch-shape autotuner: sweeps block size and XCD-aware block order of the
- * hop kernel for this lattice and precision and keeps the fastest; the table
- * persists in a text file */
+/* launch-shape autotuner: sweeps block size, XCD-aware block order and the
+ * instruction-scheduling grouping of the hop kernel for this lattice and
+ * precision and keeps the fastest; the table persists in a text file */
 int qcuTune(QcuLatticeHandle h, int prec, double *best_ms);
-int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap);
+int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group);
 int qcuTuneSave(const char *path);
 int qcuTuneLoad(const char *path);
 

@@ -42,7 +42,8 @@ _I4 = C.c_int * 4
 class SolveParam(C.Structure):
     _fields_ = [("kappa", C.c_double), ("csw", C.c_double), ("t_boundary", C.c_int),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int),
-                ("iters", C.c_int), ("schur_res", C.c_double), ("true_res", C.c_double)]
+                ("iters", C.c_int), ("schur_res", C.c_double), ("true_res", C.c_double),
+                ("setup_s", C.c_double), ("iter_s", C.c_double)]
 
 
 def _declare(L):
@@ -158,7 +159,8 @@ def bicgstab(b, U, L, kappa, csw=0.0, t_boundary=1, tol=1e-10, max_iter=10000, m
     prm = SolveParam(kappa=kappa, csw=csw, t_boundary=t_boundary, tol=tol, max_iter=max_iter,
                      mode=mode)
     lib().or_bicgstab(x, np.ascontiguousarray(b), U, _l4(L), C.byref(prm))
-    return x, {"iters": prm.iters, "schur_res": prm.schur_res, "true_res": prm.true_res}
+    return x, {"iters": prm.iters, "schur_res": prm.schur_res, "true_res": prm.true_res,
+               "setup_s": prm.setup_s, "iter_s": prm.iter_s}
 
 
 def true_residual(x, b, U, L, kappa, csw=0.0, t_boundary=1):

@@ -565,7 +565,16 @@ double or_true_residual(const ocplx *x, const ocplx *b, const ocplx *U, const in
   return den > 0 ? sqrt(num / den) : sqrt(num);
 }
 
+static double wall(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+
 int or_bicgstab(ocplx *x, const ocplx *b, const ocplx *U, const int L[4], or_solve_param *prm) {
+  double t0 = wall();
   eo_op op;
   op.U = U;
   op.L = L;
@@ -597,6 +606,8 @@ int or_bicgstab(ocplx *x, const ocplx *b, const ocplx *U, const int L[4], or_sol
   memcpy(rt, r, sizeof(ocplx) * n);
   ocplx rho_prev = 1, alpha = 1, omega = 1, rho, beta;
   double bnorm = sqrt(or_norm2(bh, n));
+  double t1 = wall();
+  prm->setup_s = t1 - t0;
   int it;
   for (it = 0; it < prm->max_iter; ++it) {
     rho = or_dot(rt, r, n);
@@ -617,6 +628,7 @@ int or_bicgstab(ocplx *x, const ocplx *b, const ocplx *U, const int L[4], or_sol
     }
   }
   prm->iters = it;
+  prm->iter_s = wall() - t1;
   /* Schur residual recomputed from x_o */
   eo_mhat(&op, w, x_o);
   double num = 0;

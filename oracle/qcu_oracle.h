@@ -69,6 +69,8 @@ typedef struct {
   int iters;        /* out */
   double schur_res; /* out: |b_hat - M_hat x_o| / |b_hat| recomputed */
   double true_res;  /* out: |b - M x| / |b| on the full lattice */
+  double setup_s;   /* out: wall seconds of the clover build/inverse and b_hat */
+  double iter_s;    /* out: wall seconds of the iterations */
 } or_solve_param;
 int or_bicgstab(ocplx *x, const ocplx *b, const ocplx *U, const int L[4],
                 or_solve_param *p);

@@ -70,7 +70,8 @@ SIGNATURES = {
     "qcuBenchHop": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.POINTER(C.c_double)]),
     "qcuTune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
-    "qcuTuneGet": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "qcuTuneGet": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int)]),
     "qcuTuneSave": (C.c_int, [C.c_char_p]),
     "qcuTuneLoad": (C.c_int, [C.c_char_p]),
 }

@@ -98,7 +98,7 @@ int legacy_dslash(void *out, void *in, void *gauge, QcuParam *param, int parity,
   QCU_TRY(spinor_ext_to_internal(in, lat->g, a.ptr(), P64, lat->stream));
   SpinorPtr none{};
   LaunchCfg cfg = tuned_cfg(*lat, P64);
-  QCU_TRY(hop(lat->op(P64), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr,
+  QCU_TRY(hop(lat->op(P64), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
               cfg, lat->stream, clover != 0));
   QCU_TRY(spinor_internal_to_ext(b.ptr(), lat->g, out, P64, lat->stream));
   QCU_HIP(hipStreamSynchronize(lat->stream));
@@ -236,7 +236,7 @@ int qcuDslash(QcuLatticeHandle h, void *out, const void *in, int parity, int clo
   QCU_TRY(b.create(prec, lat.g.vh));
   QCU_TRY(spinor_ext_to_internal(in, lat.g, a.ptr(), prec, lat.stream));
   SpinorPtr none{};
-  QCU_TRY(hop(lat.op(prec), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr,
+  QCU_TRY(hop(lat.op(prec), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
               tuned_cfg(lat, prec), lat.stream, clover != 0));
   QCU_TRY(spinor_internal_to_ext(b.ptr(), lat.g, out, prec, lat.stream));
   QCU_HIP(hipStreamSynchronize(lat.stream));
@@ -261,7 +261,7 @@ int qcuMatFull(QcuLatticeHandle h, void *out, const void *in) {
   SpinorPtr none{};
   for (int p = 0; p < 2; ++p) {
     QCU_TRY(hop(lat.op(P64), t.ptr(), x[1 - p].ptr(), p, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
-                nullptr, tuned_cfg(lat, P64), lat.stream, false));
+                tuned_cfg(lat, P64), lat.stream, false));
     QCU_TRY(cl_axpy(o.ptr(), x[p].ptr(), 0.0, none, lat.clover(p), vh, lat.stream));
     QCU_TRY(axpby(-lat.kappa, t.ptr(), P64, 1.0, o.ptr(), P64, vh, lat.stream));
     QCU_TRY(spinor_internal_to_ext(o.ptr(), lat.g, (char *)out + p * half, P64, lat.stream));
@@ -286,7 +286,7 @@ int qcuTrueResidual(QcuLatticeHandle h, const void *x, const void *b, double *re
     QCU_TRY(spinor_ext_to_internal((const char *)x + p * half, lat.g, xs[p].ptr(), P64, lat.stream));
     QCU_TRY(spinor_ext_to_internal((const char *)b + p * half, lat.g, bs[p].ptr(), P64, lat.stream));
   }
-  return lattice_true_residual(lat, xs[0].ptr(), xs[1].ptr(), bs[0].ptr(), bs[1].ptr(), res, nullptr);
+  return lattice_true_residual(lat, xs[0].ptr(), xs[1].ptr(), bs[0].ptr(), bs[1].ptr(), res, nullptr, SpinorPtr{});
 }
 
 int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p, QcuSolveInfo *info) {
@@ -327,7 +327,7 @@ int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, i
   const double k2 = lat.kappa * lat.kappa;
   auto launch = [&](int k) {
     return hop(lat.op(prec), (k & 1) ? a.ptr() : b.ptr(), (k & 1) ? b.ptr() : a.ptr(), k & 1,
-               xpay ? EPI_XPAY : EPI_PLAIN, c.ptr(), k2, RED_NONE, none, nullptr, nullptr, nullptr, cfg, lat.stream,
+               xpay ? EPI_XPAY : EPI_PLAIN, c.ptr(), k2, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
                clover != 0);
   };
   for (int k = 0; k < 3; ++k) QCU_TRY(launch(k));

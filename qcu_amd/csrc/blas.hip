@@ -77,7 +77,7 @@ QCU_D void fin_reliable(SysState *st, double rre, double rim, double rr) {
 // r = b (converted), r~ = r, p = v = 0, x = 0; |r|^2 -> state
 template <int Pb, int P, int Px>
 __global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x,
-                            SpinorPtr b, int vh, SysState *st, double *partials, unsigned *counter) {
+                            SpinorPtr b, int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0};
   if (i < vh) {
@@ -99,8 +99,7 @@ __global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v,
     for (int k = 0; k < 12; ++k) zx[k] = mk<RT<Px>>(0, 0);
     SpinorIO<Px>::store(x, i, zx);
   }
-  double tot[1];
-  if (grid_reduce<1>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) fin_norm(st, tot[0], FIN_INIT, nullptr);
+  block_partial<1>(acc, partials, blockIdx.x);
 }
 
 // p = r + beta (p - omega v)
@@ -140,7 +139,7 @@ __global__ void k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, int vh, const 
 // (P, S) = (p, s) for the plain method, (K p, K s) with a right preconditioner K
 template <int Px, int P, int PC>
 __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s,
-                            SpinorPtr t, int vh, SysState *st, double *partials, unsigned *counter) {
+                            SpinorPtr t, int vh, SysState *st, double *partials) {
   if (st->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[3] = {0, 0, 0};
@@ -182,13 +181,12 @@ __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p,
     }
     SpinorIO<P>::store(r, i, ss);
   }
-  double tot[3];
-  if (grid_reduce<3>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) fin_rho(st, tot[0], tot[1], tot[2]);
+  block_partial<3>(acc, partials, blockIdx.x);
 }
 
 // <t, s>, |t|^2 -> omega
 template <int P>
-__global__ void k_omega_dot(SpinorPtr t, SpinorPtr s, int vh, SysState *st, double *partials, unsigned *counter) {
+__global__ void k_omega_dot(SpinorPtr t, SpinorPtr s, int vh, SysState *st, double *partials) {
   if (st->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[3] = {0, 0, 0};
@@ -203,8 +201,7 @@ __global__ void k_omega_dot(SpinorPtr t, SpinorPtr s, int vh, SysState *st, doub
       acc[2] += (double)a[k].re * a[k].re + (double)a[k].im * a[k].im;
     }
   }
-  double tot[3];
-  if (grid_reduce<3>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) fin_omega(st, tot[0], tot[1], tot[2]);
+  block_partial<3>(acc, partials, blockIdx.x);
 }
 
 // y = a x + b y
@@ -236,8 +233,7 @@ __global__ void k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, int vh) {
 
 // r = b - r, |r|^2 reduced; r in precision Pr, b in Pb
 template <int Pb, int Pr>
-__global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, int vh, SysState *st, double *partials,
-                           unsigned *counter, int fin, double *out) {
+__global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0};
   if (i < vh) {
@@ -253,12 +249,11 @@ __global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, int vh, SysState *st, doubl
     }
     SpinorIO<Pr>::store(r, i, rr);
   }
-  double tot[1];
-  if (grid_reduce<1>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) fin_norm(st, tot[0], fin, out);
+  block_partial<1>(acc, partials, blockIdx.x);
 }
 
 template <int P>
-__global__ void k_norm2(SpinorPtr x, int vh, double *out, double *partials, unsigned *counter) {
+__global__ void k_norm2(SpinorPtr x, int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0};
   if (i < vh) {
@@ -267,14 +262,12 @@ __global__ void k_norm2(SpinorPtr x, int vh, double *out, double *partials, unsi
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[0] += (double)xx[k].re * xx[k].re + (double)xx[k].im * xx[k].im;
   }
-  double tot[1];
-  if (grid_reduce<1>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) *out = tot[0];
+  block_partial<1>(acc, partials, blockIdx.x);
 }
 
 // <r~, r>, |r|^2 after a reliable update
 template <int P>
-__global__ void k_reliable_dot(SpinorPtr rt, SpinorPtr r, int vh, SysState *st, double *partials,
-                               unsigned *counter) {
+__global__ void k_reliable_dot(SpinorPtr rt, SpinorPtr r, int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[3] = {0, 0, 0};
   if (i < vh) {
@@ -288,8 +281,7 @@ __global__ void k_reliable_dot(SpinorPtr rt, SpinorPtr r, int vh, SysState *st, 
       acc[2] += (double)b[k].re * b[k].re + (double)b[k].im * b[k].im;
     }
   }
-  double tot[3];
-  if (grid_reduce<3>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) fin_reliable(st, tot[0], tot[1], tot[2]);
+  block_partial<3>(acc, partials, blockIdx.x);
 }
 
 // out = C (x + a y), C the packed clover (inverse) of the parity or identity
@@ -312,33 +304,90 @@ __global__ void k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, Clo
   SpinorIO<P64>::store(out, i, xx);
 }
 
-// ------------------------------------------------------------------ launchers
-static inline int nblk(int vh, int bs) { return (vh + bs - 1) / bs; }
+// ------------------------------------------------------------------ finishing kernel
+// Sums the block partials in a fixed order and applies the epilogue.
+template <int NR>
+__global__ void __launch_bounds__(256) k_finish(const double *partials, int nblocks, int kind, SysState *st,
+                                                double *out) {
+  if (st && st->done && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO)) return;
+  double acc4[4][NR];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc4[u][k] = 0;
+  const int bs = blockDim.x;
+  int b = threadIdx.x;
+  for (; b + 3 * bs < nblocks; b += 4 * bs) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) acc4[u][k] += partials[(size_t)(b + u * bs) * NR + k];
+  }
+  for (; b < nblocks; b += bs) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc4[0][k] += partials[(size_t)b * NR + k];
+  }
+  double acc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) acc[k] = (acc4[0][k] + acc4[1][k]) + (acc4[2][k] + acc4[3][k]);
+  __shared__ double tot_sh[NR];
+  block_partial<NR>(acc, tot_sh, 0);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tot[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < NR; ++k) tot[k] = tot_sh[k];
+  switch (kind) {
+    case FK_INIT: fin_norm(st, tot[0], FIN_INIT, nullptr); break;
+    case FK_STORE:
+      for (int k = 0; k < NR; ++k) out[k] = tot[k];
+      break;
+    case FK_ACCUM:
+      for (int k = 0; k < NR; ++k) out[k] += tot[k];
+      break;
+    case FK_ALPHA: fin_alpha(st, tot[0], tot[1]); break;
+    case FK_OMEGA: fin_omega(st, tot[0], tot[1], tot[2]); break;
+    case FK_RHO: fin_rho(st, tot[0], tot[1], tot[2]); break;
+    case FK_RELIABLE: fin_reliable(st, tot[0], tot[1], tot[2]); break;
+    default: break;
+  }
+}
 
-int bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, int prec, SpinorPtr b,
-              int pb, int vh, SysState *st, double *partials, unsigned *counter, const LaunchCfg &cfg,
-              hipStream_t s) {
-  const int bs = cfg.block, nb = nblk(vh, bs);
-  // x precision: FP64 for the plain solver, FP32 for the sloppy accumulator
-  if (pb == P64 && prec == P64)
-    hipLaunchKernelGGL((k_bicg_init<P64, P64, P64>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, st,
-                       partials, counter);
-  else if (pb == P64 && prec == P32)
-    hipLaunchKernelGGL((k_bicg_init<P64, P32, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, st,
-                       partials, counter);
-  else if (pb == P64 && prec == P16)
-    hipLaunchKernelGGL((k_bicg_init<P64, P16, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, st,
-                       partials, counter);
-  else {
-    set_error(QCU_ERR_ARG, "bicg_init: unsupported precision pair");
-    return QCU_ERR_ARG;
+int finish(const double *partials, int nblocks, int nr, int kind, SysState *st, double *out, hipStream_t s) {
+  if (kind == FK_NONE) return 0;
+  switch (nr) {
+    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
+    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
+    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
+    default: set_error(QCU_ERR_ARG, "finish: bad reduction width"); return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
-int bicg_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st,
-                  const LaunchCfg &cfg, hipStream_t s) {
+// ------------------------------------------------------------------ launchers
+static inline int nblk(int vh, int bs) { return (vh + bs - 1) / bs; }
+
+int bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, int prec, SpinorPtr b, int pb, int vh,
+              SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s) {
+  const int bs = cfg.block, nb = nblk(vh, bs);
+  // x precision: FP64 for the plain solver, FP32 for the sloppy accumulator
+  if (pb == P64 && prec == P64)
+    hipLaunchKernelGGL((k_bicg_init<P64, P64, P64>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
+  else if (pb == P64 && prec == P32)
+    hipLaunchKernelGGL((k_bicg_init<P64, P32, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
+  else if (pb == P64 && prec == P16)
+    hipLaunchKernelGGL((k_bicg_init<P64, P16, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
+  else {
+    set_error(QCU_ERR_ARG, "bicg_init: unsupported precision pair");
+    return QCU_ERR_ARG;
+  }
+  QCU_HIP(hipGetLastError());
+  return finish(partials, nb, 1, FK_INIT, st, nullptr, s);
+}
+
+int bicg_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st, const LaunchCfg &cfg,
+                  hipStream_t s) {
   const int bs = cfg.block, nb = nblk(vh, bs);
   switch (prec) {
     case P64: hipLaunchKernelGGL(k_p_update<P64>, dim3(nb), dim3(bs), 0, s, p, r, v, vh, st); break;
@@ -349,8 +398,8 @@ int bicg_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, int vh, SysSt
   return 0;
 }
 
-int bicg_s_update(SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st,
-                  const LaunchCfg &cfg, hipStream_t s) {
+int bicg_s_update(SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st, const LaunchCfg &cfg,
+                  hipStream_t s) {
   const int bs = cfg.block, nb = nblk(vh, bs);
   switch (prec) {
     case P64: hipLaunchKernelGGL(k_s_update<P64>, dim3(nb), dim3(bs), 0, s, s_, r, v, vh, st); break;
@@ -363,54 +412,56 @@ int bicg_s_update(SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, int vh, SysS
 
 template <int PC>
 static int xr_launch(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s_,
-                     SpinorPtr t, int prec, int vh, SysState *st, double *partials, unsigned *counter,
-                     const LaunchCfg &cfg, hipStream_t s) {
+                     SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
+                     hipStream_t s) {
   const int bs = cfg.block, nb = nblk(vh, bs);
   if (px == P64 && prec == P64)
     hipLaunchKernelGGL((k_xr_update<P64, P64, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
-                       partials, counter);
+                       partials);
   else if (px == P32 && prec == P32)
     hipLaunchKernelGGL((k_xr_update<P32, P32, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
-                       partials, counter);
+                       partials);
   else if (px == P32 && prec == P16)
     hipLaunchKernelGGL((k_xr_update<P32, P16, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
-                       partials, counter);
+                       partials);
   else {
     set_error(QCU_ERR_ARG, "bicg_xr_update: unsupported precision pair");
     return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 3, FK_RHO, st, nullptr, s);
 }
 
-int bicg_xr_update(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr s_, SpinorPtr t,
-                   int prec, int vh, SysState *st, double *partials, unsigned *counter, const LaunchCfg &cfg,
-                   hipStream_t s) {
-  return xr_launch<0>(x, px, r, rt, p, s_, s_, t, prec, vh, st, partials, counter, cfg, s);
+int bicg_xr_update(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr s_, SpinorPtr t, int prec,
+                   int vh, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s) {
+  return xr_launch<0>(x, px, r, rt, p, s_, s_, t, prec, vh, st, partials, cfg, s);
 }
 
 int bicg_xr_update_pc(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr ph, SpinorPtr sh, SpinorPtr s_,
-                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, unsigned *counter,
-                      const LaunchCfg &cfg, hipStream_t s) {
-  return xr_launch<1>(x, px, r, rt, ph, sh, s_, t, prec, vh, st, partials, counter, cfg, s);
+                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
+                      hipStream_t s) {
+  return xr_launch<1>(x, px, r, rt, ph, sh, s_, t, prec, vh, st, partials, cfg, s);
 }
 
-int omega_dot(SpinorPtr t, SpinorPtr s_, int prec, int vh, SysState *st, double *partials, unsigned *counter,
-              hipStream_t s) {
+int omega_dot(SpinorPtr t, SpinorPtr s_, int prec, int vh, SysState *st, double *partials, hipStream_t s) {
   const int bs = 256, nb = nblk(vh, bs);
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_omega_dot<P64>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials, counter); break;
-    case P32: hipLaunchKernelGGL(k_omega_dot<P32>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials, counter); break;
-    default: hipLaunchKernelGGL(k_omega_dot<P16>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials, counter); break;
+    case P64: hipLaunchKernelGGL(k_omega_dot<P64>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
+    case P32: hipLaunchKernelGGL(k_omega_dot<P32>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
+    default: hipLaunchKernelGGL(k_omega_dot<P16>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
   }
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 3, FK_OMEGA, st, nullptr, s);
 }
 
 int axpby(double a, SpinorPtr x, int px, double b, SpinorPtr y, int py, int vh, hipStream_t s) {
   const int bs = 256, nb = nblk(vh, bs);
-#define QCU_AXPBY(A, B) \
-  if (px == A && py == B) { hipLaunchKernelGGL((k_axpby<A, B>), dim3(nb), dim3(bs), 0, s, a, x, b, y, vh); QCU_HIP(hipGetLastError()); return 0; }
+#define QCU_AXPBY(A, B)                                                                       \
+  if (px == A && py == B) {                                                                 \
+    hipLaunchKernelGGL((k_axpby<A, B>), dim3(nb), dim3(bs), 0, s, a, x, b, y, vh);          \
+    QCU_HIP(hipGetLastError());                                                             \
+    return 0;                                                                               \
+  }
   QCU_AXPBY(P64, P64)
   QCU_AXPBY(P32, P64)
   QCU_AXPBY(P16, P64)
@@ -429,34 +480,26 @@ int spinor_convert(SpinorPtr dst, int pdst, SpinorPtr src, int psrc, int vh, hip
   return axpby(1.0, src, psrc, 0.0, dst, pdst, vh, s);
 }
 
-int zero_spinor(SpinorPtr x, int px, int vh, hipStream_t s) {
-  return axpby(0.0, x, px, 0.0, x, px, vh, s);
-}
+int zero_spinor(SpinorPtr x, int px, int vh, hipStream_t s) { return axpby(0.0, x, px, 0.0, x, px, vh, s); }
 
-int xmy_norm(SpinorPtr b, int pb, SpinorPtr r, int pr, int vh, SysState *st, double *partials,
-             unsigned *counter, int fin, hipStream_t s) {
+int xmy_norm(SpinorPtr b, int pb, SpinorPtr r, int pr, int vh, SysState *st, double *partials, int fin,
+             hipStream_t s) {
   const int bs = 256, nb = nblk(vh, bs);
-  if (pb == P64 && pr == P64)
-    hipLaunchKernelGGL((k_xmy_norm<P64, P64>), dim3(nb), dim3(bs), 0, s, b, r, vh, st, partials, counter,
-                       fin, st ? st->tmp : nullptr);
-  else {
-    set_error(QCU_ERR_ARG, "xmy_norm: unsupported precision pair");
-    return QCU_ERR_ARG;
-  }
+  QCU_CHECK(pb == P64 && pr == P64, QCU_ERR_ARG, "xmy_norm: unsupported precision pair");
+  hipLaunchKernelGGL((k_xmy_norm<P64, P64>), dim3(nb), dim3(bs), 0, s, b, r, vh, partials);
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 1, fin == FIN_STORE ? FK_STORE : FK_NONE, st, st ? st->tmp : nullptr, s);
 }
 
-int reliable_dot(SpinorPtr rt, SpinorPtr r, int prec, int vh, SysState *st, double *partials,
-                 unsigned *counter, hipStream_t s) {
+int reliable_dot(SpinorPtr rt, SpinorPtr r, int prec, int vh, SysState *st, double *partials, hipStream_t s) {
   const int bs = 256, nb = nblk(vh, bs);
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_reliable_dot<P64>, dim3(nb), dim3(bs), 0, s, rt, r, vh, st, partials, counter); break;
-    case P32: hipLaunchKernelGGL(k_reliable_dot<P32>, dim3(nb), dim3(bs), 0, s, rt, r, vh, st, partials, counter); break;
-    default: hipLaunchKernelGGL(k_reliable_dot<P16>, dim3(nb), dim3(bs), 0, s, rt, r, vh, st, partials, counter); break;
+    case P64: hipLaunchKernelGGL(k_reliable_dot<P64>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
+    case P32: hipLaunchKernelGGL(k_reliable_dot<P32>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
+    default: hipLaunchKernelGGL(k_reliable_dot<P16>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
   }
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 3, FK_RELIABLE, st, nullptr, s);
 }
 
 int cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, CloverPtr C, int vh, hipStream_t s) {
@@ -466,16 +509,15 @@ int cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, CloverPtr C, int 
   return 0;
 }
 
-int norm2_dev(SpinorPtr x, int px, int vh, double *out_dev, double *partials, unsigned *counter,
-              hipStream_t s) {
+int norm2_dev(SpinorPtr x, int px, int vh, double *out_dev, double *partials, hipStream_t s) {
   const int bs = 256, nb = nblk(vh, bs);
   switch (px) {
-    case P64: hipLaunchKernelGGL(k_norm2<P64>, dim3(nb), dim3(bs), 0, s, x, vh, out_dev, partials, counter); break;
-    case P32: hipLaunchKernelGGL(k_norm2<P32>, dim3(nb), dim3(bs), 0, s, x, vh, out_dev, partials, counter); break;
-    default: hipLaunchKernelGGL(k_norm2<P16>, dim3(nb), dim3(bs), 0, s, x, vh, out_dev, partials, counter); break;
+    case P64: hipLaunchKernelGGL(k_norm2<P64>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
+    case P32: hipLaunchKernelGGL(k_norm2<P32>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
+    default: hipLaunchKernelGGL(k_norm2<P16>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
   }
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 1, FK_STORE, nullptr, out_dev, s);
 }
 
 }  // namespace qcu

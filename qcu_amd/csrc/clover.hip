@@ -10,6 +10,7 @@
 // gauge field instead of on every call.
 #include "qcu_internal.h"
 #include "qcu_reduce.h"
+#include "qcu_solver.h"
 
 namespace qcu {
 
@@ -285,7 +286,7 @@ int clover_convert(const void *c64, const Geom &g, int prec, void *dst, float *n
 // r = b - (A x - kappa t) on one parity (A packed, or identity when null);
 // |r|^2 and |b|^2 accumulated into out[0], out[1]
 __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, double kappa, int vh,
-                           double *partials, unsigned *counter, double *out) {
+                           double *partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[2] = {0, 0};
   if (i < vh) {
@@ -305,19 +306,15 @@ __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, d
       acc[1] += norm2(bb[k]);
     }
   }
-  double tot[2];
-  if (grid_reduce<2>(acc, partials, counter, blockIdx.x, gridDim.x, tot)) {
-    out[0] += tot[0];
-    out[1] += tot[1];
-  }
+  block_partial<2>(acc, partials, blockIdx.x);
 }
 
 int residual_parity(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, double kappa, int vh, double *partials,
-                    unsigned *counter, double *out, hipStream_t s) {
+                    double *out, hipStream_t s) {
   const int bs = 256, nb = (vh + bs - 1) / bs;
-  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(bs), 0, s, b, x, t, A, kappa, vh, partials, counter, out);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(bs), 0, s, b, x, t, A, kappa, vh, partials);
   QCU_HIP(hipGetLastError());
-  return 0;
+  return finish(partials, nb, 2, FK_ACCUM, nullptr, out, s);
 }
 
 }  // namespace qcu

@@ -121,10 +121,18 @@ struct HopKArgs {
   double k2;
   SysState *st;
   double *partials;
-  unsigned *counter;
 };
 
-template <int P, int CLOVER, int EPI, int RED>
+// G: scheduling groups.  0 lets the compiler hoist every load of the eight
+// directions (most memory-level parallelism, most registers); 1 and 2 fence the
+// instruction schedule after every direction / every pair of directions, which
+// caps the live registers and raises occupancy (the FP64 kernel needs this).
+#define QCU_GROUP_FENCE(G, k)                                  \
+  do {                                                         \
+    if ((G) == 1 || ((G) == 2 && ((k) & 1))) __builtin_amdgcn_sched_barrier(0); \
+  } while (0)
+
+template <int P, int CLOVER, int EPI, int RED, int G>
 __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
   typedef typename real_of<P>::type T;
   // a solver state, when given, stops the chain of kernels once the solve has stopped
@@ -143,35 +151,57 @@ __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
     for (int k = 0; k < 12; ++k) d[k] = mk<T>(0, 0);
     // x: a site with odd full x has its x-1 neighbour at the same half index
     {
-      int jb = s.xodd ? i : (s.xh == 0 ? i + g.lxh - 1 : i - 1);
-      int jf = s.xodd ? (s.xh == g.lxh - 1 ? i - g.lxh + 1 : i + 1) : i;
+      // branch-free index arithmetic keeps the body one basic block, so the
+      // scheduling fences below bound the live registers
+      const int xe = 1 - s.xodd;
+      const int jb = i - xe + xe * (int)(s.xh == 0) * g.lxh;
+      const int jf = i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh;
       hop_dir<P, 0, 0>(d, a.U, q * 4 + 0, jb, a.in, jb, (T)1);
+      QCU_GROUP_FENCE(G, 0);
       hop_dir<P, 0, 1>(d, a.U, p * 4 + 0, i, a.in, jf, (T)1);
+      QCU_GROUP_FENCE(G, 1);
     }
     {
-      int jb = s.y == 0 ? i + (g.ly - 1) * sy : i - sy;
-      int jf = s.y == g.ly - 1 ? i - (g.ly - 1) * sy : i + sy;
+      const int jb = i - sy + (int)(s.y == 0) * g.ly * sy;
+      const int jf = i + sy - (int)(s.y == g.ly - 1) * g.ly * sy;
       hop_dir<P, 1, 0>(d, a.U, q * 4 + 1, jb, a.in, jb, (T)1);
+      QCU_GROUP_FENCE(G, 2);
       hop_dir<P, 1, 1>(d, a.U, p * 4 + 1, i, a.in, jf, (T)1);
+      QCU_GROUP_FENCE(G, 3);
     }
     {
-      int jb = s.z == 0 ? i + (g.lz - 1) * sz : i - sz;
-      int jf = s.z == g.lz - 1 ? i - (g.lz - 1) * sz : i + sz;
+      const int jb = i - sz + (int)(s.z == 0) * g.lz * sz;
+      const int jf = i + sz - (int)(s.z == g.lz - 1) * g.lz * sz;
       hop_dir<P, 2, 0>(d, a.U, q * 4 + 2, jb, a.in, jb, (T)1);
+      QCU_GROUP_FENCE(G, 4);
       hop_dir<P, 2, 1>(d, a.U, p * 4 + 2, i, a.in, jf, (T)1);
+      QCU_GROUP_FENCE(G, 5);
     }
     {
-      int jb = s.t == 0 ? i + (g.lt - 1) * st_ : i - st_;
-      int jf = s.t == g.lt - 1 ? i - (g.lt - 1) * st_ : i + st_;
-      T phb = (s.t == 0) ? (T)g.tbc : (T)1;
-      T phf = (s.t == g.lt - 1) ? (T)g.tbc : (T)1;
+      const int jb = i - st_ + (int)(s.t == 0) * g.lt * st_;
+      const int jf = i + st_ - (int)(s.t == g.lt - 1) * g.lt * st_;
+      const int flip = (g.tbc < 0);
+      const T phb = (T)(1 - 2 * (flip & (int)(s.t == 0)));
+      const T phf = (T)(1 - 2 * (flip & (int)(s.t == g.lt - 1)));
       hop_dir<P, 3, 0>(d, a.U, q * 4 + 3, jb, a.in, jb, phb);
+      QCU_GROUP_FENCE(G, 6);
       hop_dir<P, 3, 1>(d, a.U, p * 4 + 3, i, a.in, jf, phf);
+      QCU_GROUP_FENCE(G, 7);
     }
     if (CLOVER) {
-      T c[72];
-      CloverIO<P>::load(a.C, i, c);
-      clover_apply_site(c, d);
+      if (P == P16) {
+        T c[72];
+        CloverIO<P>::load(a.C, i, c);
+        clover_apply_site(c, d);
+      } else {
+        // one chiral block at a time bounds the live registers
+        T c[36];
+        clover_load_block(a.C, i, 0, c);
+        clover_apply_block(c, d, 0);
+        QCU_GROUP_FENCE(G, 1);
+        clover_load_block(a.C, i, 1, c);
+        clover_apply_block(c, d, 1);
+      }
     }
     if (EPI == EPI_XPAY) {
       cplx<T> x[12];
@@ -201,19 +231,21 @@ __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
     SpinorIO<P>::store(a.out, i, d);
   }
   if (RED == RED_DOT_Y_OUT) {
-    double v[2] = {red[0], red[1]}, tot[2];
-    if (grid_reduce<2>(v, a.partials, a.counter, blockIdx.x, gridDim.x, tot)) fin_alpha(a.st, tot[0], tot[1]);
+    double v[2] = {red[0], red[1]};
+    block_partial<2>(v, a.partials, blockIdx.x);
   } else if (RED == RED_OUT_X) {
-    double tot[3];
-    if (grid_reduce<3>(red, a.partials, a.counter, blockIdx.x, gridDim.x, tot))
-      fin_omega(a.st, tot[0], tot[1], tot[2]);
+    block_partial<3>(red, a.partials, blockIdx.x);
   }
 }
 
 template <int P, int CLOVER, int EPI, int RED>
 static int launch(const HopKArgs &a, const LaunchCfg &cfg, hipStream_t s) {
   int nb = (a.g.vh + cfg.block - 1) / cfg.block;
-  hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED>), dim3(nb), dim3(cfg.block), 0, s, a);
+  switch (cfg.group) {
+    case 1: hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED, 1>), dim3(nb), dim3(cfg.block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED, 2>), dim3(nb), dim3(cfg.block), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED, 0>), dim3(nb), dim3(cfg.block), 0, s, a); break;
+  }
   QCU_HIP(hipGetLastError());
   return 0;
 }
@@ -236,12 +268,12 @@ static int dispatch(const HopKArgs &a, int clover, int epi, int red, const Launc
 }
 
 int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, SpinorPtr x, double k2,
-        int red, SpinorPtr y, SysState *st, double *partials, unsigned *counter, const LaunchCfg &cfg,
-        hipStream_t s, bool use_clover) {
+        int red, SpinorPtr y, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s,
+        bool use_clover) {
   QCU_CHECK(parity == 0 || parity == 1, QCU_ERR_ARG, "parity must be 0 or 1");
-  QCU_CHECK(cfg.block >= 64 && cfg.block <= 1024 && cfg.block % 64 == 0, QCU_ERR_ARG,
-            "block size must be a multiple of 64 in [64, 1024]");
-  QCU_CHECK(red == RED_NONE || (epi == EPI_XPAY && st && partials && counter), QCU_ERR_ARG,
+  QCU_CHECK(cfg.block >= 64 && cfg.block <= 256 && cfg.block % 64 == 0, QCU_ERR_ARG,
+            "block size must be a multiple of 64 in [64, 256]");
+  QCU_CHECK(red == RED_NONE || (epi == EPI_XPAY && st && partials), QCU_ERR_ARG,
             "fused reductions need the xpay epilogue and solver state");
   HopKArgs a;
   a.out = out;
@@ -256,15 +288,18 @@ int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, Spino
   a.k2 = k2;
   a.st = st;
   a.partials = partials;
-  a.counter = counter;
   int clover = use_clover && op.C[parity].p != nullptr;
+  int rc;
   switch (op.prec) {
-    case P64: return dispatch<P64>(a, clover, epi, red, cfg, s);
-    case P32: return dispatch<P32>(a, clover, epi, red, cfg, s);
-    case P16: return dispatch<P16>(a, clover, epi, red, cfg, s);
+    case P64: rc = dispatch<P64>(a, clover, epi, red, cfg, s); break;
+    case P32: rc = dispatch<P32>(a, clover, epi, red, cfg, s); break;
+    case P16: rc = dispatch<P16>(a, clover, epi, red, cfg, s); break;
+    default: set_error(QCU_ERR_ARG, "unknown precision"); return QCU_ERR_ARG;
   }
-  set_error(QCU_ERR_ARG, "unknown precision");
-  return QCU_ERR_ARG;
+  if (rc || red == RED_NONE) return rc;
+  const int nb = (op.g.vh + cfg.block - 1) / cfg.block;
+  return red == RED_DOT_Y_OUT ? finish(partials, nb, 2, FK_ALPHA, st, nullptr, s)
+                              : finish(partials, nb, 3, FK_OMEGA, st, nullptr, s);
 }
 
 }  // namespace qcu

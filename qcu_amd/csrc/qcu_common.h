@@ -95,6 +95,19 @@ QCU_D Site decode(const Geom &g, int i, int parity) {
   return s;
 }
 
+// Loads and stores address a wavefront-uniform base (component k of a field,
+// kept in scalar registers) plus a 32-bit per-lane byte offset, so that the
+// compiler can use the scalar-base / 32-bit-offset form of global memory
+// instructions instead of a 64-bit address per load and per component.
+template <class V>
+QCU_D V ld_at(const void *base, size_t comp_bytes, unsigned lane_off) {
+  return *(const V *)((const char *)base + comp_bytes + lane_off);
+}
+template <class V>
+QCU_D void st_at(void *base, size_t comp_bytes, unsigned lane_off, const V &v) {
+  *(V *)((char *)base + comp_bytes + lane_off) = v;
+}
+
 // ---------------------------------------------------------------------------
 // precision tags
 // ---------------------------------------------------------------------------
@@ -123,22 +136,17 @@ template <>
 struct SpinorIO<P64> {
   typedef double real;
   static QCU_D void load(const SpinorPtr &f, int i, cplx<double> (&v)[12]) {
-    const double2 *p = (const double2 *)f.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      double2 a = p[(size_t)k * f.stride];
+      double2 a = ld_at<double2>(f.p, (size_t)k * f.stride * 16, off);
       v[k] = mk(a.x, a.y);
     }
   }
   static QCU_D void store(const SpinorPtr &f, int i, const cplx<double> (&v)[12]) {
-    double2 *p = (double2 *)f.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      double2 a;
-      a.x = v[k].re;
-      a.y = v[k].im;
-      p[(size_t)k * f.stride] = a;
-    }
+    for (int k = 0; k < 12; ++k) st_at<double2>(f.p, (size_t)k * f.stride * 16, off, make_double2(v[k].re, v[k].im));
   }
 };
 
@@ -146,25 +154,20 @@ template <>
 struct SpinorIO<P32> {
   typedef float real;
   static QCU_D void load(const SpinorPtr &f, int i, cplx<float> (&v)[12]) {
-    const float4 *p = (const float4 *)f.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      float4 a = p[(size_t)k * f.stride];
+      float4 a = ld_at<float4>(f.p, (size_t)k * f.stride * 16, off);
       v[2 * k] = mk(a.x, a.y);
       v[2 * k + 1] = mk(a.z, a.w);
     }
   }
   static QCU_D void store(const SpinorPtr &f, int i, const cplx<float> (&v)[12]) {
-    float4 *p = (float4 *)f.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      float4 a;
-      a.x = v[2 * k].re;
-      a.y = v[2 * k].im;
-      a.z = v[2 * k + 1].re;
-      a.w = v[2 * k + 1].im;
-      p[(size_t)k * f.stride] = a;
-    }
+    for (int k = 0; k < 6; ++k)
+      st_at<float4>(f.p, (size_t)k * f.stride * 16, off,
+                    make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im));
   }
 };
 
@@ -175,11 +178,11 @@ template <>
 struct SpinorIO<P16> {
   typedef float real;
   static QCU_D void load(const SpinorPtr &f, int i, cplx<float> (&v)[12]) {
-    const uint4 *p = (const uint4 *)f.p + i;
-    float nrm = f.norm[i];
+    const unsigned off = (unsigned)i * 16u;
+    const float nrm = ld_at<float>(f.norm, 0, (unsigned)i * 4u);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      uint4 a = p[(size_t)k * f.stride];
+      uint4 a = ld_at<uint4>(f.p, (size_t)k * f.stride * 16, off);
       unsigned w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -193,16 +196,16 @@ struct SpinorIO<P16> {
 #pragma unroll
     for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
     float inv = m > 0.f ? 1.0f / m : 0.f;
-    uint4 *p = (uint4 *)f.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       unsigned w[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         w[q] = (unsigned)f2h(v[4 * k + q].re * inv) | ((unsigned)f2h(v[4 * k + q].im * inv) << 16);
-      p[(size_t)k * f.stride] = make_uint4(w[0], w[1], w[2], w[3]);
+      st_at<uint4>(f.p, (size_t)k * f.stride * 16, off, make_uint4(w[0], w[1], w[2], w[3]));
     }
-    f.norm[i] = m;
+    st_at<float>(f.norm, 0, (unsigned)i * 4u, m);
   }
 };
 
@@ -230,10 +233,10 @@ template <>
 struct LinkIO<P64> {
   // (p, mu) slab index pm = p*4 + mu
   static QCU_D void load(const LinkPtr &L, int pm, int i, cplx<double> (&u)[9]) {
-    const double2 *q = (const double2 *)L.a + (size_t)pm * 6 * L.vh + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      double2 a = q[(size_t)k * L.vh];
+      double2 a = ld_at<double2>(L.a, ((size_t)pm * 6 + k) * L.vh * 16, off);
       u[k] = mk(a.x, a.y);
     }
     recon_row2(u);
@@ -243,10 +246,10 @@ struct LinkIO<P64> {
 template <>
 struct LinkIO<P32> {
   static QCU_D void load(const LinkPtr &L, int pm, int i, cplx<float> (&u)[9]) {
-    const float4 *q = (const float4 *)L.a + (size_t)pm * 3 * L.vh + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      float4 a = q[(size_t)k * L.vh];
+      float4 a = ld_at<float4>(L.a, ((size_t)pm * 3 + k) * L.vh * 16, off);
       u[2 * k] = mk(a.x, a.y);
       u[2 * k + 1] = mk(a.z, a.w);
     }
@@ -257,8 +260,8 @@ struct LinkIO<P32> {
 template <>
 struct LinkIO<P16> {
   static QCU_D void load(const LinkPtr &L, int pm, int i, cplx<float> (&u)[9]) {
-    uint4 a = ((const uint4 *)L.a)[(size_t)pm * L.vh + i];
-    uint2 b = ((const uint2 *)L.b)[(size_t)pm * L.vh + i];
+    uint4 a = ld_at<uint4>(L.a, (size_t)pm * L.vh * 16, (unsigned)i * 16u);
+    uint2 b = ld_at<uint2>(L.b, (size_t)pm * L.vh * 8, (unsigned)i * 8u);
     unsigned w[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -284,10 +287,10 @@ struct CloverIO;
 template <>
 struct CloverIO<P64> {
   static QCU_D void load(const CloverPtr &C, int i, double (&c)[72]) {
-    const double2 *q = (const double2 *)C.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 36; ++k) {
-      double2 a = q[(size_t)k * C.vh];
+      double2 a = ld_at<double2>(C.p, (size_t)k * C.vh * 16, off);
       c[2 * k] = a.x;
       c[2 * k + 1] = a.y;
     }
@@ -296,10 +299,10 @@ struct CloverIO<P64> {
 template <>
 struct CloverIO<P32> {
   static QCU_D void load(const CloverPtr &C, int i, float (&c)[72]) {
-    const float4 *q = (const float4 *)C.p + i;
+    const unsigned off = (unsigned)i * 16u;
 #pragma unroll
     for (int k = 0; k < 18; ++k) {
-      float4 a = q[(size_t)k * C.vh];
+      float4 a = ld_at<float4>(C.p, (size_t)k * C.vh * 16, off);
       c[4 * k] = a.x;
       c[4 * k + 1] = a.y;
       c[4 * k + 2] = a.z;
@@ -310,11 +313,11 @@ struct CloverIO<P32> {
 template <>
 struct CloverIO<P16> {
   static QCU_D void load(const CloverPtr &C, int i, float (&c)[72]) {
-    const uint4 *q = (const uint4 *)C.p + i;
-    float nrm = C.norm[i];
+    const unsigned off = (unsigned)i * 16u;
+    const float nrm = ld_at<float>(C.norm, 0, (unsigned)i * 4u);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      uint4 a = q[(size_t)k * C.vh];
+      uint4 a = ld_at<uint4>(C.p, (size_t)k * C.vh * 16, off);
       unsigned w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
       for (int q2 = 0; q2 < 4; ++q2) {
@@ -325,32 +328,57 @@ struct CloverIO<P16> {
   }
 };
 
-// out = Ainv * v for one site; block b covers spins {2b, 2b+1}
+// out_b = C_b * v_b for chiral block b (spins {2b, 2b+1}) from its 36 packed reals
 template <class T>
-QCU_D void clover_apply_site(const T (&c)[72], cplx<T> (&v)[12]) {
-  cplx<T> o[12];
+QCU_D void clover_apply_block(const T *cb, cplx<T> (&v)[12], int b) {
+  cplx<T> o[6];
+  const cplx<T> *x = v + 6 * b;
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const T *cb = c + 36 * b;
-    const cplx<T> *x = v + 6 * b;
+  for (int r = 0; r < 6; ++r) {
+    cplx<T> acc = mk(cb[r] * x[r].re, cb[r] * x[r].im);
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      cplx<T> acc = mk(cb[r] * x[r].re, cb[r] * x[r].im);
-#pragma unroll
-      for (int col = 0; col < 6; ++col) {
-        if (col < r) {
-          int k = 6 + 2 * (r * (r - 1) / 2 + col);
-          cmad(acc, mk(cb[k], cb[k + 1]), x[col]);
-        } else if (col > r) {
-          int k = 6 + 2 * (col * (col - 1) / 2 + r);
-          cmadj(acc, mk(cb[k], cb[k + 1]), x[col]);
-        }
+    for (int col = 0; col < 6; ++col) {
+      if (col < r) {
+        int k = 6 + 2 * (r * (r - 1) / 2 + col);
+        cmad(acc, mk(cb[k], cb[k + 1]), x[col]);
+      } else if (col > r) {
+        int k = 6 + 2 * (col * (col - 1) / 2 + r);
+        cmadj(acc, mk(cb[k], cb[k + 1]), x[col]);
       }
-      o[6 * b + r] = acc;
     }
+    o[r] = acc;
   }
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = o[k];
+  for (int r = 0; r < 6; ++r) v[6 * b + r] = o[r];
+}
+
+// out = C * v for one site (both chiral blocks)
+template <class T>
+QCU_D void clover_apply_site(const T (&c)[72], cplx<T> (&v)[12]) {
+  clover_apply_block(c, v, 0);
+  clover_apply_block(c + 36, v, 1);
+}
+
+// load one chiral block (36 reals) of a packed FP64 / FP32 clover field
+QCU_D void clover_load_block(const CloverPtr &C, int i, int b, double (&c)[36]) {
+  const unsigned off = (unsigned)i * 16u;
+#pragma unroll
+  for (int k = 0; k < 18; ++k) {
+    double2 a = ld_at<double2>(C.p, (size_t)(18 * b + k) * C.vh * 16, off);
+    c[2 * k] = a.x;
+    c[2 * k + 1] = a.y;
+  }
+}
+QCU_D void clover_load_block(const CloverPtr &C, int i, int b, float (&c)[36]) {
+  const unsigned off = (unsigned)i * 16u;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    float4 a = ld_at<float4>(C.p, (size_t)(9 * b + k) * C.vh * 16, off);
+    c[4 * k] = a.x;
+    c[4 * k + 1] = a.y;
+    c[4 * k + 2] = a.z;
+    c[4 * k + 3] = a.w;
+  }
 }
 
 // XCD-aware block remap: hardware places workgroup b on XCD b % 8; give each

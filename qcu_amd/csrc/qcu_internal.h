@@ -102,6 +102,7 @@ struct SysState {
 struct LaunchCfg {
   int block = 256;
   int remap = 1;
+  int group = 0;  // hop scheduling groups (see dslash.hip)
 };
 
 // layout.hip
@@ -124,8 +125,8 @@ enum HopRed { RED_NONE = 0, RED_DOT_Y_OUT = 1, RED_OUT_X = 2 };
 // out = x - k2 [A_p^-1] D_{p,1-p} in                           (EPI_XPAY)
 // RED_DOT_Y_OUT: <y,out> -> alpha update; RED_OUT_X: <out,x>,|out|^2 -> omega
 int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, SpinorPtr x, double k2,
-        int red, SpinorPtr y, SysState *st, double *partials, unsigned *counter, const LaunchCfg &cfg,
-        hipStream_t s, bool use_clover = true);
+        int red, SpinorPtr y, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s,
+        bool use_clover = true);
 
 // clover.hip
 // packed A (clov, optional) and A^-1 (cinv) of one parity from FP64 internal links
@@ -134,30 +135,32 @@ int clover_build(const LinkPtr &U64, const Geom &g, double coeff, int parity, vo
 int clover_convert(const void *c64, const Geom &g, int prec, void *dst, float *norm, hipStream_t s);
 // out[0] += |b - (A x - kappa t)|^2, out[1] += |b|^2 over one parity
 int residual_parity(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, double kappa, int vh,
-                    double *partials, unsigned *counter, double *out, hipStream_t s);
+                    double *partials, double *out, hipStream_t s);
 
 // blas.hip
+// one-block kernel: fixed-order sum of nblocks partials of width nr, then the
+// epilogue `kind` (FinKindAll in qcu_reduce.h)
+int finish(const double *partials, int nblocks, int nr, int kind, SysState *st, double *out, hipStream_t s);
 int bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, int prec,
-              SpinorPtr b, int pb, int vh, SysState *st, double *partials, unsigned *counter,
-              const LaunchCfg &cfg, hipStream_t s);
+              SpinorPtr b, int pb, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
+              hipStream_t s);
 int bicg_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st,
                   const LaunchCfg &cfg, hipStream_t s);
 int bicg_s_update(SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st,
                   const LaunchCfg &cfg, hipStream_t s);
 int bicg_xr_update(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr s_,
-                   SpinorPtr t, int prec, int vh, SysState *st, double *partials, unsigned *counter,
+                   SpinorPtr t, int prec, int vh, SysState *st, double *partials,
                    const LaunchCfg &cfg, hipStream_t s);
 // y = a*x + b*y  (real a,b, mixed precisions allowed)
 int axpby(double a, SpinorPtr x, int px, double b, SpinorPtr y, int py, int vh, hipStream_t s);
-// r = b - r ; returns |r|^2 into st->tmp[0] via reduction
-int xmy_norm(SpinorPtr b, int pb, SpinorPtr r, int pr, int vh, SysState *st, double *partials,
-             unsigned *counter, int fin, hipStream_t s);
+// r = b - r; with FIN_STORE |r|^2 lands in st->tmp[0]
+int xmy_norm(SpinorPtr b, int pb, SpinorPtr r, int pr, int vh, SysState *st, double *partials, int fin,
+             hipStream_t s);
 int reliable_dot(SpinorPtr rt, SpinorPtr r, int prec, int vh, SysState *st, double *partials,
-                 unsigned *counter, hipStream_t s);
+                 hipStream_t s);
 // out = C (x + a y) in FP64, C a packed clover matrix of one parity or identity (C.p == null)
 int cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, CloverPtr C, int vh, hipStream_t s);
-int norm2_dev(SpinorPtr x, int px, int vh, double *out_dev, double *partials, unsigned *counter,
-              hipStream_t s);
+int norm2_dev(SpinorPtr x, int px, int vh, double *out_dev, double *partials, hipStream_t s);
 int zero_spinor(SpinorPtr x, int px, int vh, hipStream_t s);
 
 // gen.hip

@@ -24,9 +24,8 @@ struct Lattice {
   DevBuf clov64[2];        // packed A (for the true residual)
   DevBuf cinv64[2];        // packed A^-1
   DevBuf cinv32[2], cinv16[2], cinv16n[2];
-  DevBuf scratch;          // partials + counters + state + bad flag
+  DevBuf scratch;          // partials + state + small results + bad flag
   double *partials = nullptr;
-  unsigned *counters = nullptr;  // kNumCounters re-armed counters
   SysState *state = nullptr;     // kNumStates solver states
   double *dscal = nullptr;       // small FP64 results
   int *bad = nullptr;
@@ -35,7 +34,6 @@ struct Lattice {
 
   std::unique_ptr<Workspace> ws;
 
-  static constexpr int kNumCounters = 64;
   static constexpr int kNumStates = 64;
 
   int init(const int L_[4], double kappa_, double csw_, int tbc);
@@ -52,8 +50,10 @@ struct Lattice {
 };
 
 // vectors in the external layout, helpers for the C entry points
+// scratch: an FP64 half-field work vector, or {} to allocate one for the call
 int lattice_true_residual(Lattice &lat, SpinorPtr xe, SpinorPtr xo, SpinorPtr be, SpinorPtr bo,
-                          double *res, double *bnorm);
+                          double *res, double *bnorm, SpinorPtr scratch);
+LaunchCfg tuned_cfg(const Lattice &lat, int prec);
 
 // device-local error string used by the legacy C entry points
 int current_device();

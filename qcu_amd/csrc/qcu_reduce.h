@@ -1,11 +1,12 @@
-// qcu_reduce.h -- deterministic FP64 block reductions with a fused epilogue.
+// qcu_reduce.h -- deterministic FP64 grid reductions.
 //
-// Each block reduces its lanes with 64-wide wavefront shuffles, the waves of a
-// block through LDS, and writes one partial per block.  The block that takes
-// the last ticket of an agent-scope counter sums all partials in a fixed order
-// (so the result does not depend on scheduling) and runs the epilogue (scalar
-// updates of the solver) in one thread, then re-arms the counter.  No second
-// kernel launch and no host round trip are needed for alpha, omega, beta.
+// Each block reduces its lanes with 64-wide wavefront shuffles and the waves
+// of the block through LDS, and writes one partial per block.  A one-block
+// finishing kernel (finish() in blas.hip) then sums the partials in a fixed
+// order -- so results do not depend on scheduling -- and runs the scalar
+// epilogue of the solver (alpha, omega, rho/beta updates) on the device.  The
+// kernel boundary orders the two; no device-wide atomics are used (a counter
+// shared by thousands of blocks serialises at the memory side of an MI355X).
 #pragma once
 #include "qcu_common.h"
 
@@ -19,21 +20,11 @@ QCU_D double wave_sum(double v) {
   return v;
 }
 
-QCU_D double load_agent(const double *p) {
-  unsigned long long u =
-      __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
-// Reduces NR values over the grid.  Returns true in thread 0 of the last block,
-// with the grid totals in tot.  Must be called by every thread of the block.
-// nblocks is the number of blocks taking part (gridDim.x for 1-D grids) and
-// slot the index of this block's partial.
+// Sums v over the block and stores it as partials[slot * NR + k].  Must be
+// called by every thread of the block.
 template <int NR>
-QCU_D bool grid_reduce(double (&v)[NR], double *partials, unsigned *counter, int slot, int nblocks,
-                       double (&tot)[NR]) {
+QCU_D void block_partial(const double (&v)[NR], double *partials, int slot) {
   __shared__ double sh[kMaxWaves][NR];
-  __shared__ int last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
@@ -49,37 +40,19 @@ QCU_D bool grid_reduce(double (&v)[NR], double *partials, unsigned *counter, int
       for (int w = 0; w < nw; ++w) s += sh[w][k];
       partials[(size_t)slot * NR + k] = s;
     }
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == (unsigned)(nblocks - 1));
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // fixed-order sum of the partials by the last block
-  double acc[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) acc[k] = 0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < NR; ++k) acc[k] += load_agent(partials + (size_t)b * NR + k);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    double w = wave_sum(acc[k]);
-    if (lane == 0) sh[wave][k] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return false;
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    double s = 0;
-    for (int w = 0; w < nw; ++w) s += sh[w][k];
-    tot[k] = s;
-  }
-  __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
+
+// epilogues run by the finishing kernel on the grid totals
+enum FinKindAll {
+  FK_NONE = 0,
+  FK_INIT = 1,      // BiCGSTAB start: r~ = r, rho = |r|^2
+  FK_STORE = 3,     // out[k] = tot[k]
+  FK_ACCUM = 4,     // out[k] += tot[k]
+  FK_ALPHA = 5,     // alpha = rho / <r~, v>
+  FK_OMEGA = 6,     // omega = <t, s> / <t, t>
+  FK_RHO = 7,       // rho, |r|^2, beta
+  FK_RELIABLE = 8,  // after a reliable update
+};
 
 }  // namespace qcu

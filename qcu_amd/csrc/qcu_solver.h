@@ -45,11 +45,10 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
 int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam &prm, QcuSolveInfo *info);
 
 // blas.hip, preconditioned variants
-int omega_dot(SpinorPtr t, SpinorPtr s, int prec, int vh, SysState *st, double *partials, unsigned *counter,
-              hipStream_t strm);
+int omega_dot(SpinorPtr t, SpinorPtr s, int prec, int vh, SysState *st, double *partials, hipStream_t strm);
 int bicg_xr_update_pc(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr ph, SpinorPtr sh, SpinorPtr s,
-                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, unsigned *counter,
-                      const LaunchCfg &cfg, hipStream_t strm);
+                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
+                      hipStream_t strm);
 
 }  // namespace qcu
 

@@ -87,8 +87,7 @@ int Lattice::init(const int L_[4], double kappa_, double csw_, int tbc) {
   QCU_CHECK(device >= 0, QCU_ERR_STATE, "no HIP device");
   QCU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   const size_t nb = max_blocks();
-  const size_t bytes = nb * 4 * sizeof(double) + kNumCounters * sizeof(unsigned) +
-                       kNumStates * sizeof(SysState) + 64 * sizeof(double) + 64;
+  const size_t bytes = nb * 4 * sizeof(double) + kNumStates * sizeof(SysState) + 64 * sizeof(double) + 64;
   QCU_TRY(scratch.alloc(bytes + 1024));
   char *base = (char *)scratch.p;
   partials = (double *)base;
@@ -97,8 +96,6 @@ int Lattice::init(const int L_[4], double kappa_, double csw_, int tbc) {
   base += kNumStates * sizeof(SysState);
   dscal = (double *)base;
   base += 64 * sizeof(double);
-  counters = (unsigned *)base;
-  base += kNumCounters * sizeof(unsigned);
   bad = (int *)base;
   QCU_HIP(hipMemsetAsync(scratch.p, 0, bytes, stream));
   QCU_HIP(hipHostMalloc((void **)&host_state, kNumStates * sizeof(SysState), hipHostMallocDefault));
@@ -209,22 +206,26 @@ Lattice::~Lattice() {
 
 // |b - M x| / |b| on the full lattice, M = A - kappa D, x and b as internal FP64 parities
 int lattice_true_residual(Lattice &lat, SpinorPtr xe, SpinorPtr xo, SpinorPtr be, SpinorPtr bo, double *res,
-                          double *bnorm) {
-  Spinor t;
-  QCU_TRY(t.create(P64, lat.g.vh));
+                          double *bnorm, SpinorPtr scratch) {
+  Spinor own;
+  SpinorPtr t = scratch;
+  if (!t.p) {
+    QCU_TRY(own.create(P64, lat.g.vh));
+    t = own.ptr();
+  }
   HopOp op = lat.op(P64);
-  LaunchCfg cfg;
+  LaunchCfg cfg = tuned_cfg(lat, P64);
   QCU_HIP(hipMemsetAsync(lat.dscal, 0, 2 * sizeof(double), lat.stream));
   SpinorPtr none{};
   // even: t = D_eo x_o
-  QCU_TRY(hop(op, t.ptr(), xo, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr, cfg,
-              lat.stream, false));
-  QCU_TRY(residual_parity(be, xe, t.ptr(), lat.clover(0), lat.kappa, lat.g.vh, lat.partials, lat.counters,
-                          lat.dscal, lat.stream));
-  QCU_TRY(hop(op, t.ptr(), xe, 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr, cfg,
-              lat.stream, false));
-  QCU_TRY(residual_parity(bo, xo, t.ptr(), lat.clover(1), lat.kappa, lat.g.vh, lat.partials, lat.counters,
-                          lat.dscal, lat.stream));
+  QCU_TRY(hop(op, t, xo, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
+              false));
+  QCU_TRY(residual_parity(be, xe, t, lat.clover(0), lat.kappa, lat.g.vh, lat.partials, lat.dscal,
+                          lat.stream));
+  QCU_TRY(hop(op, t, xe, 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
+              false));
+  QCU_TRY(residual_parity(bo, xo, t, lat.clover(1), lat.kappa, lat.g.vh, lat.partials, lat.dscal,
+                          lat.stream));
   QCU_HIP(hipMemcpyAsync(lat.host_scal, lat.dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, lat.stream));
   QCU_HIP(hipStreamSynchronize(lat.stream));
   const double num = lat.host_scal[0], den = lat.host_scal[1];

@@ -11,6 +11,8 @@
 // iterations.  A chunk is captured once as a hipGraph and replayed.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "../../include/qcu.h"
@@ -18,6 +20,21 @@
 #include "qcu_solver.h"
 
 namespace qcu {
+
+// QCU_TRACE=1 prints the wall time of every solver phase to stderr
+struct PhaseTrace {
+  bool on;
+  hipStream_t s;
+  std::chrono::steady_clock::time_point t;
+  explicit PhaseTrace(hipStream_t st) : on(getenv("QCU_TRACE") != nullptr), s(st), t(std::chrono::steady_clock::now()) {}
+  void mark(const char *what) {
+    if (!on) return;
+    (void)hipStreamSynchronize(s);
+    auto n = std::chrono::steady_clock::now();
+    fprintf(stderr, "[qcu] %-28s %9.3f ms\n", what, std::chrono::duration<double, std::milli>(n - t).count());
+    t = n;
+  }
+};
 
 int VecSet::create(int prec_, int px_, int vh, bool precond) {
   prec = prec_;
@@ -75,8 +92,8 @@ static int apply_mhat64(Lattice &lat, SpinorPtr out, SpinorPtr x, SpinorPtr tmp)
   HopOp op = lat.op(P64);
   LaunchCfg cfg = tuned_cfg(lat, P64);
   SpinorPtr none{};
-  QCU_TRY(hop(op, tmp, x, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr, cfg, lat.stream));
-  QCU_TRY(hop(op, out, tmp, 1, EPI_XPAY, x, lat.kappa * lat.kappa, RED_NONE, none, nullptr, nullptr, nullptr,
+  QCU_TRY(hop(op, tmp, x, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream));
+  QCU_TRY(hop(op, out, tmp, 1, EPI_XPAY, x, lat.kappa * lat.kappa, RED_NONE, none, nullptr, nullptr,
               cfg, lat.stream));
   return 0;
 }
@@ -90,15 +107,14 @@ static int enqueue_iteration(Lattice &lat, VecSet &S, SysState *st, Precond *K) 
   bcfg.block = 256;
   SpinorPtr none{};
   const double k2 = lat.kappa * lat.kappa;
-  unsigned *ctr = lat.counters;
   QCU_TRY(bicg_p_update(S.p.ptr(), S.r.ptr(), S.v.ptr(), prec, vh, st, bcfg, lat.stream));
   SpinorPtr pin = S.p.ptr();
   if (K) {
     QCU_TRY(K->apply(lat, S.ph.ptr(), S.p.ptr(), prec, st));
     pin = S.ph.ptr();
   }
-  QCU_TRY(hop(op, S.tmp.ptr(), pin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, nullptr, cfg, lat.stream));
-  QCU_TRY(hop(op, S.v.ptr(), S.tmp.ptr(), 1, EPI_XPAY, pin, k2, RED_DOT_Y_OUT, S.rt.ptr(), st, lat.partials, ctr,
+  QCU_TRY(hop(op, S.tmp.ptr(), pin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, cfg, lat.stream));
+  QCU_TRY(hop(op, S.v.ptr(), S.tmp.ptr(), 1, EPI_XPAY, pin, k2, RED_DOT_Y_OUT, S.rt.ptr(), st, lat.partials,
               cfg, lat.stream));
   QCU_TRY(bicg_s_update(S.s.ptr(), S.r.ptr(), S.v.ptr(), prec, vh, st, bcfg, lat.stream));
   SpinorPtr sin = S.s.ptr();
@@ -106,19 +122,19 @@ static int enqueue_iteration(Lattice &lat, VecSet &S, SysState *st, Precond *K) 
     QCU_TRY(K->apply(lat, S.sh.ptr(), S.s.ptr(), prec, st));
     sin = S.sh.ptr();
   }
-  QCU_TRY(hop(op, S.tmp.ptr(), sin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, nullptr, cfg, lat.stream));
+  QCU_TRY(hop(op, S.tmp.ptr(), sin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, cfg, lat.stream));
   if (K) {
     // t = Mhat s_hat; <t, s> needs s, not s_hat: reduce in the update kernel below
-    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_NONE, none, st, nullptr, nullptr, cfg,
+    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_NONE, none, st, nullptr, cfg,
                 lat.stream));
-    QCU_TRY(omega_dot(S.t.ptr(), S.s.ptr(), prec, vh, st, lat.partials, ctr, lat.stream));
+    QCU_TRY(omega_dot(S.t.ptr(), S.s.ptr(), prec, vh, st, lat.partials, lat.stream));
     QCU_TRY(bicg_xr_update_pc(S.x.ptr(), S.px, S.r.ptr(), S.rt.ptr(), S.ph.ptr(), S.sh.ptr(), S.s.ptr(),
-                              S.t.ptr(), prec, vh, st, lat.partials, ctr, bcfg, lat.stream));
+                              S.t.ptr(), prec, vh, st, lat.partials, bcfg, lat.stream));
   } else {
-    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_OUT_X, S.s.ptr(), st, lat.partials, ctr,
+    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_OUT_X, S.s.ptr(), st, lat.partials,
                 cfg, lat.stream));
     QCU_TRY(bicg_xr_update(S.x.ptr(), S.px, S.r.ptr(), S.rt.ptr(), S.p.ptr(), S.s.ptr(), S.t.ptr(), prec, vh,
-                           st, lat.partials, ctr, bcfg, lat.stream));
+                           st, lat.partials, bcfg, lat.stream));
   }
   return 0;
 }
@@ -185,7 +201,7 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
   SpinorPtr src = ws.bh.ptr();
   if (warm) {
     QCU_TRY(apply_mhat64(lat, ws.r64.ptr(), ws.xo.ptr(), ws.t64.ptr()));
-    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, lat.counters, FIN_NONE,
+    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, FIN_NONE,
                      lat.stream));
     src = ws.r64.ptr();
   } else {
@@ -193,8 +209,10 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
   }
   // S.x = 0, r = src, r~ = r, p = v = 0
   QCU_TRY(bicg_init(S->r.ptr(), S->rt.ptr(), S->p.ptr(), S->v.ptr(), S->x.ptr(), prec, src, P64, vh, st,
-                    lat.partials, lat.counters, bcfg, lat.stream));
+                    lat.partials, bcfg, lat.stream));
   QCU_TRY(read_state(lat, st, hs));
+  PhaseTrace trace(lat.stream);
+  trace.mark("  init");
   int iters = 0, reliable = 0, stuck = 0, restarts = 0;
   bool converged = false;
   double last_true = hs->rr;
@@ -205,6 +223,7 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
       QCU_TRY(run_chunk(lat, *S, st, K, n, use_graph && n == chunk));
       QCU_TRY(read_state(lat, st, hs));
       iters = hs->iter;
+      trace.mark("  chunk");
       continue;
     }
     const int why = hs->done;
@@ -214,7 +233,7 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
     QCU_TRY(axpby(1.0, S->x.ptr(), S->px, 1.0, ws.xo.ptr(), P64, vh, lat.stream));
     QCU_TRY(zero_spinor(S->x.ptr(), S->px, vh, lat.stream));
     QCU_TRY(apply_mhat64(lat, ws.r64.ptr(), ws.xo.ptr(), ws.t64.ptr()));
-    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, lat.counters, FIN_STORE,
+    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, FIN_STORE,
                      lat.stream));
     QCU_TRY(read_state(lat, st, hs));
     const double rr_true = hs->tmp[0];
@@ -251,8 +270,9 @@ int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_it
     }
     hs->done = 0;
     QCU_TRY(write_state(lat, st, hs));
-    QCU_TRY(reliable_dot(S->rt.ptr(), S->r.ptr(), prec, vh, st, lat.partials, lat.counters, lat.stream));
+    QCU_TRY(reliable_dot(S->rt.ptr(), S->r.ptr(), prec, vh, st, lat.partials, lat.stream));
     QCU_TRY(read_state(lat, st, hs));
+    trace.mark("  reliable update");
   }
   QCU_TRY(axpby(1.0, S->x.ptr(), S->px, 1.0, ws.xo.ptr(), P64, vh, lat.stream));
   QCU_TRY(zero_spinor(S->x.ptr(), S->px, vh, lat.stream));
@@ -272,6 +292,7 @@ int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam
   QCU_TRY(ws.ensure(lat.g.vh));
   const int vh = lat.g.vh;
   auto t0 = std::chrono::steady_clock::now();
+  PhaseTrace trace(lat.stream);
   const char *b = (const char *)b_ext;
   char *x = (char *)x_ext;
   const size_t half = (size_t)vh * 12 * 16;
@@ -282,17 +303,18 @@ int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam
   LaunchCfg cfg = tuned_cfg(lat, P64);
   SpinorPtr none{};
   QCU_TRY(cl_axpy(ws.t64.ptr(), ws.be.ptr(), 0.0, none, lat.clover_inv(P64, 0), vh, lat.stream));
-  QCU_TRY(hop(op, ws.r64.ptr(), ws.t64.ptr(), 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr, cfg,
+  QCU_TRY(hop(op, ws.r64.ptr(), ws.t64.ptr(), 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg,
               lat.stream, false));
   QCU_TRY(cl_axpy(ws.bh.ptr(), ws.bo.ptr(), lat.kappa, ws.r64.ptr(), lat.clover_inv(P64, 1), vh, lat.stream));
   // |b|^2 for the stopping rule
-  QCU_TRY(norm2_dev(ws.be.ptr(), P64, vh, lat.dscal, lat.partials, lat.counters, lat.stream));
-  QCU_TRY(norm2_dev(ws.bo.ptr(), P64, vh, lat.dscal + 1, lat.partials, lat.counters, lat.stream));
-  QCU_TRY(norm2_dev(ws.bh.ptr(), P64, vh, lat.dscal + 2, lat.partials, lat.counters, lat.stream));
+  QCU_TRY(norm2_dev(ws.be.ptr(), P64, vh, lat.dscal, lat.partials, lat.stream));
+  QCU_TRY(norm2_dev(ws.bo.ptr(), P64, vh, lat.dscal + 1, lat.partials, lat.stream));
+  QCU_TRY(norm2_dev(ws.bh.ptr(), P64, vh, lat.dscal + 2, lat.partials, lat.stream));
   QCU_HIP(hipMemcpyAsync(lat.host_scal, lat.dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, lat.stream));
   QCU_HIP(hipStreamSynchronize(lat.stream));
   const double b2 = lat.host_scal[0] + lat.host_scal[1];
   const double bh2 = lat.host_scal[2];
+  trace.mark("b_hat + norms");
   double target2;
   if (prm.legacy_stop)
     target2 = prm.tol;  // Re<r,r> < tol on the Schur residual, as the reference
@@ -308,6 +330,7 @@ int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam
   for (int attempt = 0; attempt < 6; ++attempt) {
     QCU_TRY(schur_solve(lat, prm.precision, target2, delta, max_iter - total_iters, chunk, prm.use_graph != 0, K,
                         attempt > 0, &st));
+    trace.mark("schur solve");
     total_iters += st.iters;
     total_rel += st.reliable;
     restarts += st.restarts;
@@ -318,10 +341,12 @@ int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam
       continue;
     }
     // x_e = Ae^-1 (b_e + kappa D_eo x_o)
-    QCU_TRY(hop(op, ws.t64.ptr(), ws.xo.ptr(), 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, nullptr, cfg,
+    QCU_TRY(hop(op, ws.t64.ptr(), ws.xo.ptr(), 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg,
                 lat.stream, false));
     QCU_TRY(cl_axpy(ws.xe.ptr(), ws.be.ptr(), lat.kappa, ws.t64.ptr(), lat.clover_inv(P64, 0), vh, lat.stream));
-    QCU_TRY(lattice_true_residual(lat, ws.xe.ptr(), ws.xo.ptr(), ws.be.ptr(), ws.bo.ptr(), &res, nullptr));
+    QCU_TRY(lattice_true_residual(lat, ws.xe.ptr(), ws.xo.ptr(), ws.be.ptr(), ws.bo.ptr(), &res, nullptr,
+                                  ws.r64.ptr()));
+    trace.mark("x_e + true residual");
     if (prm.legacy_stop || res <= prm.tol || total_iters >= max_iter || !st.converged) break;
     target2 *= 0.01;  // tighten the Schur target and continue from the current x_o
   }

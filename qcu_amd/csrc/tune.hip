@@ -52,19 +52,29 @@ int qcuTune(QcuLatticeHandle h, int prec, double *best_ms) {
   QCU_CHECK(h, QCU_ERR_ARG, "null handle");
   Lattice &lat = h->lat;
   const int clover = lat.has_clover() ? 1 : 0;
-  const int blocks[] = {64, 128, 256, 512};
+  const int blocks[] = {64, 128, 256};
   LaunchCfg best;
   double bms = 1e30;
-  for (int b : blocks)
-    for (int r = 0; r < 2; ++r) {
-      double ms = 0;
-      QCU_TRY(qcuBenchHop(h, prec, clover, 1, 20, b, r, &ms));
-      if (ms < bms) {
-        bms = ms;
-        best.block = b;
-        best.remap = r;
+  for (int grp = 0; grp < 3; ++grp)
+    for (int b : blocks)
+      for (int r = 0; r < 2; ++r) {
+        {
+          std::lock_guard<std::mutex> lk(g_tune_mu);
+          LaunchCfg c;
+          c.block = b;
+          c.remap = r;
+          c.group = grp;
+          g_tuned[key_of(lat, prec, clover)] = c;
+        }
+        double ms = 0;
+        QCU_TRY(qcuBenchHop(h, prec, clover, 1, 20, b, r, &ms));
+        if (ms < bms) {
+          bms = ms;
+          best.block = b;
+          best.remap = r;
+          best.group = grp;
+        }
       }
-    }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key_of(lat, prec, clover)] = best;
@@ -73,11 +83,12 @@ int qcuTune(QcuLatticeHandle h, int prec, double *best_ms) {
   return 0;
 }
 
-int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap) {
-  QCU_CHECK(h && block && remap, QCU_ERR_ARG, "null argument");
+int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group) {
+  QCU_CHECK(h && block && remap && group, QCU_ERR_ARG, "null argument");
   LaunchCfg c = tuned_cfg(h->lat, prec);
   *block = c.block;
   *remap = c.remap;
+  *group = c.group;
   return 0;
 }
 
@@ -86,7 +97,8 @@ int qcuTuneSave(const char *path) {
   FILE *f = fopen(path, "w");
   QCU_CHECK(f, QCU_ERR_ARG, std::string("cannot write ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  for (auto &kv : g_tuned) fprintf(f, "%s %d %d\n", kv.first.c_str(), kv.second.block, kv.second.remap);
+  for (auto &kv : g_tuned)
+    fprintf(f, "%s %d %d %d\n", kv.first.c_str(), kv.second.block, kv.second.remap, kv.second.group);
   fclose(f);
   return 0;
 }
@@ -96,13 +108,14 @@ int qcuTuneLoad(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return 0;  // no cache yet
   char key[160];
-  int b, r;
+  int b, r, grp;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  while (fscanf(f, "%159s %d %d", key, &b, &r) == 3) {
-    if (b >= 64 && b <= 1024 && b % 64 == 0 && (r == 0 || r == 1)) {
+  while (fscanf(f, "%159s %d %d %d", key, &b, &r, &grp) == 4) {
+    if (b >= 64 && b <= 256 && b % 64 == 0 && (r == 0 || r == 1) && grp >= 0 && grp <= 2) {
       LaunchCfg c;
       c.block = b;
       c.remap = r;
+      c.group = grp;
       g_tuned[key] = c;
     }
   }

@@ -1,0 +1,65 @@
+"""Hop-kernel microbenchmark (profiling driver).
+
+    python -m qcu_amd.hopbench --prec 2 --clover 1 --xpay 1 --iters 50
+    python -m qcu_amd.hopbench --copy-mib 1024 --iters 50   # plain copy of known size
+
+Runs the resident hop kernel on a 32^3x64 clover lattice (eps 0.2, csw 1.0,
+kappa 0.1248) and prints the HIP-event time per launch and the algorithmic
+GB/s.  The copy mode launches a device-to-device copy of a known byte count,
+used to calibrate the FETCH_SIZE / WRITE_SIZE counters.
+"""
+import argparse
+import json
+
+HOP = {0: 8 * 192 + 8 * 96 + 192, 1: 8 * 96 + 8 * 48 + 96, 2: 8 * 52 + 8 * 24 + 52}
+CLOVER = {0: 576, 1: 288, 2: 148}
+XPAY = {0: 192, 1: 96, 2: 52}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--L", type=int, nargs=4, default=[32, 32, 32, 64])
+    ap.add_argument("--prec", type=int, default=2)
+    ap.add_argument("--clover", type=int, default=1)
+    ap.add_argument("--xpay", type=int, default=1)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--remap", type=int, default=-1)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    args = ap.parse_args()
+
+    import torch
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20)
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.iters):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.iters
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "ms": ms,
+                          "GBps_rw": 2 * n / ms / 1e6}))
+        return
+
+    from .lattice import Lattice, random_gauge
+    L = tuple(args.L)
+    U = random_gauge(L, 0.2, 1)
+    lat = Lattice(L, 0.1248, 1.0 if args.clover else 0.0, -1)
+    lat.load_gauge(U)
+    del U
+    torch.cuda.empty_cache()
+    ms = lat.bench_hop(prec=args.prec, clover=bool(args.clover), xpay=bool(args.xpay), iters=args.iters,
+                       block=args.block, remap=args.remap)
+    alg = (HOP[args.prec] + (CLOVER[args.prec] if args.clover else 0) +
+           (XPAY[args.prec] if args.xpay else 0)) * lat.vh
+    print(json.dumps({"L": L, "prec": args.prec, "clover": args.clover, "xpay": args.xpay,
+                      "ms_per_launch": ms, "alg_bytes": alg, "alg_GBps": alg / ms / 1e6}))
+
+
+if __name__ == "__main__":
+    main()

@@ -148,9 +148,9 @@ class Lattice:
     def tune(self, prec=FP64):
         ms = C.c_double()
         _lib.check(self._lib.qcuTune(self.h, int(prec), C.byref(ms)))
-        b, r = C.c_int(), C.c_int()
-        _lib.check(self._lib.qcuTuneGet(self.h, int(prec), C.byref(b), C.byref(r)))
-        return {"block": b.value, "remap": r.value, "ms": ms.value}
+        b, r, g = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(self._lib.qcuTuneGet(self.h, int(prec), C.byref(b), C.byref(r), C.byref(g)))
+        return {"block": b.value, "remap": r.value, "group": g.value, "ms": ms.value}
 
 
 def tune_save(path):
