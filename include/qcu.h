@@ -122,6 +122,22 @@ int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p,
 int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap,
                 double *ms);
 
+/* ---- batches (new) -------------------------------------------------------
+ * Many systems in one launch per kernel: nrhs[k] right-hand sides on lattice
+ * lats[k] (interleaved so that they share every link load), for nlat
+ * lattices of any shapes.  System order: lattice 0 rhs 0..nrhs[0]-1, then
+ * lattice 1, ...; x[i], b[i] are the full-lattice external fields of system i. */
+typedef struct QcuBatch_s *QcuBatchHandle;
+int qcuBatchCreate(QcuBatchHandle *h, const QcuLatticeHandle *lats, const int *nrhs, int nlat);
+int qcuBatchDestroy(QcuBatchHandle h);
+int qcuBatchSystems(QcuBatchHandle h, int *nsys);
+/* info: one record per system */
+int qcuBatchSolve(QcuBatchHandle h, void *const *x, const void *const *b, const QcuSolveParam *p,
+                  QcuSolveInfo *info);
+/* ms per launch of the batched hop kernel (all systems, one parity) */
+int qcuBatchBenchHop(QcuBatchHandle h, int prec, int xpay, int iters, double *ms);
+int qcuBatchTune(QcuBatchHandle h, int prec, double *best_ms);
+
 /* launch-shape autotuner: sweeps block size, XCD-aware block order and the
  * instruction-scheduling grouping of the hop kernel for this lattice and
  * precision and keeps the fastest; the table persists in a text file */

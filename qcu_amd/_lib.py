@@ -70,6 +70,13 @@ SIGNATURES = {
     "qcuBenchHop": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.POINTER(C.c_double)]),
     "qcuTune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
+    "qcuBatchCreate": (C.c_int, [C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_int), C.c_int]),
+    "qcuBatchDestroy": (C.c_int, [_VP]),
+    "qcuBatchSystems": (C.c_int, [_VP, C.POINTER(C.c_int)]),
+    "qcuBatchSolve": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(SolveParam),
+                                C.POINTER(SolveInfo)]),
+    "qcuBatchBenchHop": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "qcuBatchTune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "qcuTuneGet": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_int)]),
     "qcuTuneSave": (C.c_int, [C.c_char_p]),

@@ -18,16 +18,22 @@
 
 namespace qcu {
 const char *last_error_msg();
-int tune_lookup(const Lattice &lat, int prec, LaunchCfg *cfg);
+int bench_hop(const BatchOp &op, hipStream_t s, int prec, int clover, int xpay, int iters, const LaunchCfg &cfg,
+              double *ms);
 }  // namespace qcu
 
 using namespace qcu;
 
-namespace {
+struct QcuBatch_s {
+  hipStream_t stream = nullptr;
+  BatchOp op;
+  Solver solver;
+  ~QcuBatch_s() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
 
-std::mutex g_legacy_mu;
-// resident lattices of the legacy entry points, one per (device, kind)
-std::map<std::pair<int, int>, std::unique_ptr<Lattice>> g_legacy;
+namespace {
 
 int check_dev_ptr(const void *p, const char *what) {
   if (!p) {
@@ -63,20 +69,29 @@ int check_grid(const QcuParam *grid) {
   return 0;
 }
 
-// kind 0: Wilson (kappa 0.125), kind 1: clover with kappa*csw = 1
-Lattice *legacy_lattice(const QcuParam *param, int kind, int *rc) {
-  int dev = current_device();
+// resident lattices of the legacy entry points, one per (device, kind):
+// kind 0 Wilson with kappa 0.125, kind 1 clover with kappa*csw = 1
+struct Legacy {
+  Lattice lat;
+  Spinor a, b;
+};
+std::mutex g_legacy_mu;
+std::map<std::pair<int, int>, std::unique_ptr<Legacy>> g_legacy;
+
+Legacy *legacy_ctx(const QcuParam *param, int kind, int *rc) {
+  const int dev = current_device();
   auto key = std::make_pair(dev, kind);
   auto it = g_legacy.find(key);
   if (it != g_legacy.end()) {
-    Lattice *l = it->second.get();
-    if (std::memcmp(l->L, param->lattice_size, sizeof(l->L)) == 0) return l;
+    if (std::memcmp(it->second->lat.L, param->lattice_size, sizeof(int) * 4) == 0) return it->second.get();
     g_legacy.erase(it);
   }
-  std::unique_ptr<Lattice> l(new Lattice());
-  *rc = kind == 0 ? l->init(param->lattice_size, 0.125, 0.0, 1) : l->init(param->lattice_size, 1.0, 1.0, 1);
+  std::unique_ptr<Legacy> l(new Legacy());
+  *rc = kind == 0 ? l->lat.init(param->lattice_size, 0.125, 0.0, 1) : l->lat.init(param->lattice_size, 1.0, 1.0, 1);
+  if (!*rc) *rc = l->a.create(P64, l->lat.g.vh);
+  if (!*rc) *rc = l->b.create(P64, l->lat.g.vh);
   if (*rc) return nullptr;
-  Lattice *raw = l.get();
+  Legacy *raw = l.get();
   g_legacy[key] = std::move(l);
   return raw;
 }
@@ -89,19 +104,26 @@ int legacy_dslash(void *out, void *in, void *gauge, QcuParam *param, int parity,
   QCU_TRY(check_dev_ptr(gauge, "gauge"));
   std::lock_guard<std::mutex> lk(g_legacy_mu);
   int rc = 0;
-  Lattice *lat = legacy_lattice(param, clover, &rc);
-  if (!lat) return rc;
-  QCU_TRY(lat->load_gauge(gauge, 1));
-  Spinor a, b;
-  QCU_TRY(a.create(P64, lat->g.vh));
-  QCU_TRY(b.create(P64, lat->g.vh));
-  QCU_TRY(spinor_ext_to_internal(in, lat->g, a.ptr(), P64, lat->stream));
+  Legacy *L = legacy_ctx(param, clover, &rc);
+  if (!L) return rc;
+  Lattice &lat = L->lat;
+  QCU_TRY(lat.load_gauge(gauge, 1));
+  QCU_TRY(spinor_ext_to_batch(in, lat.self, 0, 0, L->a.ptr(), P64, lat.stream));
   SpinorPtr none{};
-  LaunchCfg cfg = tuned_cfg(*lat, P64);
-  QCU_TRY(hop(lat->op(P64), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
-              cfg, lat->stream, clover != 0));
-  QCU_TRY(spinor_internal_to_ext(b.ptr(), lat->g, out, P64, lat->stream));
-  QCU_HIP(hipStreamSynchronize(lat->stream));
+  QCU_TRY(hop(lat.self, P64, L->b.ptr(), L->a.ptr(), parity, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr,
+              tuned_cfg(lat.self, P64), lat.stream, clover != 0));
+  QCU_TRY(spinor_batch_to_ext(L->b.ptr(), lat.self, 0, 0, out, P64, lat.stream));
+  QCU_HIP(hipStreamSynchronize(lat.stream));
+  return 0;
+}
+
+int lattice_solver(Lattice &lat, Solver **S) {
+  if (!lat.solver) {
+    std::unique_ptr<Solver> s(new Solver());
+    QCU_TRY(s->init(&lat.self, lat.stream));
+    lat.solver = std::move(s);
+  }
+  *S = lat.solver.get();
   return 0;
 }
 
@@ -112,19 +134,22 @@ int legacy_bicgstab(void *x, void *b, void *gauge, QcuParam *param) {
   QCU_TRY(check_dev_ptr(gauge, "gauge"));
   std::lock_guard<std::mutex> lk(g_legacy_mu);
   int rc = 0;
-  Lattice *lat = legacy_lattice(param, 0, &rc);
-  if (!lat) return rc;
-  QCU_TRY(lat->load_gauge(gauge, 1));
+  Legacy *L = legacy_ctx(param, 0, &rc);
+  if (!L) return rc;
+  QCU_TRY(L->lat.load_gauge(gauge, 1));
+  Solver *S = nullptr;
+  QCU_TRY(lattice_solver(L->lat, &S));
   QcuSolveParam prm{};
-  prm.tol = 1e-9;         // Re<r,r> < 1e-9 (include/define.h:68)
-  prm.max_iter = 1000;    // include/define.h:67
+  prm.tol = 1e-9;       // Re<r,r> < 1e-9 (include/define.h:68)
+  prm.max_iter = 1000;  // include/define.h:67
   prm.precision = P64;
   prm.chunk = 8;
   prm.use_graph = 1;
   prm.legacy_stop = 1;
   QcuSolveInfo info{};
-  QCU_TRY(full_solve(*lat, x, b, prm, &info));
-  return 0;
+  void *xs[1] = {x};
+  const void *bs[1] = {b};
+  return full_solve(*S, xs, bs, prm, &info);
 }
 
 }  // namespace
@@ -227,18 +252,18 @@ int qcuDslash(QcuLatticeHandle h, void *out, const void *in, int parity, int clo
   Lattice &lat = h->lat;
   QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
   QCU_CHECK(parity == 0 || parity == 1, QCU_ERR_ARG, "parity must be 0 or 1");
-  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
+  QCU_CHECK(prec >= 0 && prec <= 2 && ((lat.prec_mask >> prec) & 1), QCU_ERR_ARG, "precision not resident");
   QCU_CHECK(!clover || lat.has_clover(), QCU_ERR_STATE, "lattice has no clover term (csw = 0)");
   QCU_TRY(check_dev_ptr(out, "out"));
   QCU_TRY(check_dev_ptr(in, "in"));
   Spinor a, b;
   QCU_TRY(a.create(prec, lat.g.vh));
   QCU_TRY(b.create(prec, lat.g.vh));
-  QCU_TRY(spinor_ext_to_internal(in, lat.g, a.ptr(), prec, lat.stream));
+  QCU_TRY(spinor_ext_to_batch(in, lat.self, 0, 0, a.ptr(), prec, lat.stream));
   SpinorPtr none{};
-  QCU_TRY(hop(lat.op(prec), b.ptr(), a.ptr(), parity, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
-              tuned_cfg(lat, prec), lat.stream, clover != 0));
-  QCU_TRY(spinor_internal_to_ext(b.ptr(), lat.g, out, prec, lat.stream));
+  QCU_TRY(hop(lat.self, prec, b.ptr(), a.ptr(), parity, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr,
+              tuned_cfg(lat.self, prec), lat.stream, clover != 0));
+  QCU_TRY(spinor_batch_to_ext(b.ptr(), lat.self, 0, 0, out, prec, lat.stream));
   QCU_HIP(hipStreamSynchronize(lat.stream));
   return 0;
 }
@@ -250,6 +275,7 @@ int qcuMatFull(QcuLatticeHandle h, void *out, const void *in) {
   QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
   QCU_TRY(check_dev_ptr(out, "out"));
   QCU_TRY(check_dev_ptr(in, "in"));
+  const BatchOp &op = lat.self;
   const int vh = lat.g.vh;
   const size_t half = (size_t)vh * 12 * 16;
   Spinor x[2], t, o;
@@ -257,14 +283,14 @@ int qcuMatFull(QcuLatticeHandle h, void *out, const void *in) {
   QCU_TRY(t.create(P64, vh));
   QCU_TRY(o.create(P64, vh));
   for (int p = 0; p < 2; ++p)
-    QCU_TRY(spinor_ext_to_internal((const char *)in + p * half, lat.g, x[p].ptr(), P64, lat.stream));
+    QCU_TRY(spinor_ext_to_batch((const char *)in + p * half, op, 0, 0, x[p].ptr(), P64, lat.stream));
   SpinorPtr none{};
   for (int p = 0; p < 2; ++p) {
-    QCU_TRY(hop(lat.op(P64), t.ptr(), x[1 - p].ptr(), p, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr,
-                tuned_cfg(lat, P64), lat.stream, false));
-    QCU_TRY(cl_axpy(o.ptr(), x[p].ptr(), 0.0, none, lat.clover(p), vh, lat.stream));
-    QCU_TRY(axpby(-lat.kappa, t.ptr(), P64, 1.0, o.ptr(), P64, vh, lat.stream));
-    QCU_TRY(spinor_internal_to_ext(o.ptr(), lat.g, (char *)out + p * half, P64, lat.stream));
+    QCU_TRY(hop(op, P64, t.ptr(), x[1 - p].ptr(), p, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr,
+                tuned_cfg(op, P64), lat.stream, false));
+    QCU_TRY(cl_axpy(op, o.ptr(), x[p].ptr(), 0.0, false, none, 2, p, lat.stream));
+    QCU_TRY(axpby(op, -lat.kappa, t.ptr(), P64, 1.0, o.ptr(), P64, lat.stream));
+    QCU_TRY(spinor_batch_to_ext(o.ptr(), op, 0, 0, (char *)out + p * half, P64, lat.stream));
   }
   QCU_HIP(hipStreamSynchronize(lat.stream));
   return 0;
@@ -277,16 +303,10 @@ int qcuTrueResidual(QcuLatticeHandle h, const void *x, const void *b, double *re
   QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
   QCU_TRY(check_dev_ptr(x, "x"));
   QCU_TRY(check_dev_ptr(b, "b"));
-  const int vh = lat.g.vh;
-  const size_t half = (size_t)vh * 12 * 16;
-  Spinor xs[2], bs[2];
-  for (int p = 0; p < 2; ++p) {
-    QCU_TRY(xs[p].create(P64, vh));
-    QCU_TRY(bs[p].create(P64, vh));
-    QCU_TRY(spinor_ext_to_internal((const char *)x + p * half, lat.g, xs[p].ptr(), P64, lat.stream));
-    QCU_TRY(spinor_ext_to_internal((const char *)b + p * half, lat.g, bs[p].ptr(), P64, lat.stream));
-  }
-  return lattice_true_residual(lat, xs[0].ptr(), xs[1].ptr(), bs[0].ptr(), bs[1].ptr(), res, nullptr, SpinorPtr{});
+  Solver *S = nullptr;
+  QCU_TRY(lattice_solver(lat, &S));
+  const void *xs[1] = {x}, *bs[1] = {b};
+  return true_residuals(*S, xs, bs, res);
 }
 
 int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p, QcuSolveInfo *info) {
@@ -298,52 +318,85 @@ int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p,
   QCU_TRY(check_dev_ptr(b, "b"));
   Lattice &lat = h->lat;
   QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
-  QCU_CHECK(p->precision == P64 || (lat.prec_mask >> p->precision) & 1, QCU_ERR_STATE,
-            "links of the requested precision are not resident");
-  return full_solve(lat, x, b, *p, info);
+  QCU_CHECK((lat.prec_mask >> p->precision) & 1, QCU_ERR_STATE, "links of the requested precision are not resident");
+  Solver *S = nullptr;
+  QCU_TRY(lattice_solver(lat, &S));
+  void *xs[1] = {x};
+  const void *bs[1] = {b};
+  QcuSolveInfo tmp{};
+  return full_solve(*S, xs, bs, *p, info ? info : &tmp);
 }
 
 int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap, double *ms) {
   clear_error();
   QCU_CHECK(h && ms, QCU_ERR_ARG, "null argument");
-  QCU_CHECK(prec >= 0 && prec <= 2 && iters > 0, QCU_ERR_ARG, "bad precision or iteration count");
   Lattice &lat = h->lat;
   QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
   QCU_CHECK(!clover || lat.has_clover(), QCU_ERR_STATE, "lattice has no clover term");
-  const int vh = lat.g.vh;
-  Spinor a, b, c;
-  QCU_TRY(a.create(prec, vh));
-  QCU_TRY(b.create(prec, vh));
-  QCU_TRY(c.create(prec, vh));
-  Spinor g64;
-  QCU_TRY(g64.create(P64, vh));
-  QCU_TRY(gen_gauss(g64.data.p, (long)vh * 12, 77, lat.stream));
-  QCU_TRY(spinor_convert(a.ptr(), prec, g64.ptr(), P64, vh, lat.stream));
-  QCU_TRY(spinor_convert(c.ptr(), prec, g64.ptr(), P64, vh, lat.stream));
-  LaunchCfg cfg = tuned_cfg(lat, prec);
+  LaunchCfg cfg = tuned_cfg(lat.self, prec);
   if (block > 0) cfg.block = block;
   if (remap >= 0) cfg.remap = remap;
-  SpinorPtr none{};
-  const double k2 = lat.kappa * lat.kappa;
-  auto launch = [&](int k) {
-    return hop(lat.op(prec), (k & 1) ? a.ptr() : b.ptr(), (k & 1) ? b.ptr() : a.ptr(), k & 1,
-               xpay ? EPI_XPAY : EPI_PLAIN, c.ptr(), k2, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
-               clover != 0);
-  };
-  for (int k = 0; k < 3; ++k) QCU_TRY(launch(k));
-  hipEvent_t e0, e1;
-  QCU_HIP(hipEventCreate(&e0));
-  QCU_HIP(hipEventCreate(&e1));
-  QCU_HIP(hipEventRecord(e0, lat.stream));
-  for (int k = 0; k < iters; ++k) QCU_TRY(launch(k));
-  QCU_HIP(hipEventRecord(e1, lat.stream));
-  QCU_HIP(hipEventSynchronize(e1));
-  float t = 0;
-  QCU_HIP(hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *ms = (double)t / iters;
+  return bench_hop(lat.self, lat.stream, prec, clover, xpay, iters, cfg, ms);
+}
+
+// ------------------------------------------------------------------ batches
+int qcuBatchCreate(QcuBatchHandle *h, const QcuLatticeHandle *lats, const int *nrhs, int nlat) {
+  clear_error();
+  QCU_CHECK(h && lats && nrhs && nlat > 0, QCU_ERR_ARG, "null argument");
+  std::unique_ptr<QcuBatch_s> b(new QcuBatch_s());
+  QCU_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  std::vector<const Lattice *> ls;
+  std::vector<int> nr;
+  for (int k = 0; k < nlat; ++k) {
+    QCU_CHECK(lats[k], QCU_ERR_ARG, "null lattice handle");
+    ls.push_back(&lats[k]->lat);
+    nr.push_back(nrhs[k]);
+  }
+  QCU_TRY(b->op.build(ls, nr, b->stream));
+  QCU_TRY(b->solver.init(&b->op, b->stream));
+  *h = b.release();
   return 0;
 }
 
+int qcuBatchDestroy(QcuBatchHandle h) {
+  clear_error();
+  delete h;
+  return 0;
+}
+
+int qcuBatchSystems(QcuBatchHandle h, int *nsys) {
+  clear_error();
+  QCU_CHECK(h && nsys, QCU_ERR_ARG, "null argument");
+  *nsys = h->op.nsys;
+  return 0;
+}
+
+int qcuBatchSolve(QcuBatchHandle h, void *const *x, const void *const *b, const QcuSolveParam *p,
+                  QcuSolveInfo *info) {
+  clear_error();
+  QCU_CHECK(h && x && b && p, QCU_ERR_ARG, "null argument");
+  QCU_CHECK(p->precision >= 0 && p->precision <= 2, QCU_ERR_ARG, "bad precision");
+  QCU_CHECK(p->tol > 0, QCU_ERR_ARG, "tolerance must be positive");
+  QCU_CHECK(p->precond == nullptr, QCU_ERR_ARG, "batched solves take no preconditioner");
+  for (const Lattice *l : h->op.lats)
+    QCU_CHECK((l->prec_mask >> p->precision) & 1, QCU_ERR_STATE, "links of the requested precision are not resident");
+  for (int k = 0; k < h->op.nsys; ++k) {
+    QCU_TRY(check_dev_ptr(x[k], "x"));
+    QCU_TRY(check_dev_ptr(b[k], "b"));
+  }
+  std::vector<QcuSolveInfo> tmp(h->op.nsys);
+  return full_solve(h->solver, x, b, *p, info ? info : tmp.data());
+}
+
+int qcuBatchBenchHop(QcuBatchHandle h, int prec, int xpay, int iters, double *ms) {
+  clear_error();
+  QCU_CHECK(h && ms, QCU_ERR_ARG, "null argument");
+  return bench_hop(h->op, h->stream, prec, h->op.has_clover, xpay, iters, tuned_cfg(h->op, prec), ms);
+}
+
 }  // extern "C"
+
+namespace qcu {
+const BatchOp &batch_op(const QcuBatch_s *b) { return b->op; }
+hipStream_t batch_stream(const QcuBatch_s *b) { return b->stream; }
+}  // namespace qcu

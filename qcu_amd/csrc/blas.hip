@@ -1,12 +1,15 @@
-// blas.hip -- fused BLAS-1 site kernels of the BiCGSTAB recurrence.
+// blas.hip -- fused BLAS-1 site kernels of the BiCGSTAB recurrence over a
+// batch of systems (segment tables, qcu_batch.h).
 //
-// Each kernel touches every vector once and reduces in FP64 with the fused
-// grid reduction of qcu_reduce.h; the scalar epilogues of qcu_state.h keep
+// Each kernel touches every vector once; reductions are per system, in FP64,
+// in two deterministic stages (block partials, then the one-block-per-system
+// finishing kernel k_finish), and the scalar epilogues of qcu_state.h keep
 // rho, alpha, beta, omega on the device (the reference reads them back
 // through cuBLAS and 1-thread kernels, src/bistabcg.cu:4-180,
 // include/lattice_bistabcg.h:149-163).  Vectors may be FP64, FP32 or FP16 with
 // a per-site scale; arithmetic is FP32 for the sloppy precisions and FP64
-// otherwise, and every global sum is FP64.
+// otherwise.  A system whose state says it has stopped is skipped.
+#include "qcu_batch.h"
 #include "qcu_internal.h"
 #include "qcu_reduce.h"
 #include "qcu_solver.h"
@@ -28,14 +31,49 @@ QCU_D void cvt(const cplx<RT<Ps>> (&a)[12], cplx<RT<Pd>> (&b)[12]) {
   for (int k = 0; k < 12; ++k) b[k] = mk<RT<Pd>>((RT<Pd>)a[k].re, (RT<Pd>)a[k].im);
 }
 
+template <int P>
+QCU_D void zero12(cplx<RT<P>> (&v)[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = mk<RT<P>>(0, 0);
+}
 
-QCU_D void fin_norm(SysState *st, double rr, int kind, double *out) {
-  if (kind == FIN_NONE) return;
-  if (kind == FIN_STORE) {
-    *out = rr;
-    return;
-  }
-  // FIN_INIT: r~ = r = b_hat, x = 0, p = v = 0
+// per-thread view of a batch: segment, local virtual site, system
+struct TCtx {
+  const SegDesc *sd;
+  int t, t0, sys, nrhs;
+  bool valid;   // inside a segment
+  bool active;  // valid and its system still running (or no state given)
+};
+
+QCU_D TCtx tctx(const SegDesc *segs, int nseg, const SysState *st) {
+  TCtx c;
+  const BlockSeg bs = find_seg(segs, nseg, blockIdx.x, blockDim.x);
+  c.valid = false;
+  c.active = false;
+  c.sd = nullptr;
+  c.t0 = bs.t0;
+  c.nrhs = 1;
+  c.t = 0;
+  c.sys = 0;
+  if (bs.seg < 0) return c;
+  c.sd = segs + bs.seg;
+  c.nrhs = c.sd->nrhs;
+  c.t = bs.t0 + threadIdx.x;
+  c.sys = c.sd->sys0 + c.t % c.nrhs;
+  c.valid = c.t < c.sd->nthr;
+  c.active = c.valid && (!st || st[c.sys].done == 0);
+  return c;
+}
+
+template <int P>
+QCU_D SpinorPtr view(const SpinorPtr &f, const TCtx &c) {
+  return seg_view(f, P, c.sd->voff, c.sd->nthr);
+}
+
+// ---------------------------------------------------------------- epilogues
+// start of a solve: r~ = r = source, rho = |r|^2, alpha = omega = rho_prev = 1
+QCU_D void fin_init(SysState *st, double rr) {
+  if (st->done == 4) return;  // a system left out of this solve
   st->rho_prev[0] = 1;
   st->rho_prev[1] = 0;
   st->alpha[0] = 1;
@@ -74,88 +112,85 @@ QCU_D void fin_reliable(SysState *st, double rre, double rim, double rr) {
   st->done = (rr < st->rr_stop) ? 1 : 0;
 }
 
-// r = b (converted), r~ = r, p = v = 0, x = 0; |r|^2 -> state
+// ---------------------------------------------------------------- kernels
+// r = b (converted), r~ = r, p = v = 0, x = 0; |r|^2 per system
 template <int Pb, int P, int Px>
-__global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x,
-                            SpinorPtr b, int vh, double *partials) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, SpinorPtr b,
+                            const SegDesc *segs, int nseg, double *partials) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.sd) return;
   double acc[1] = {0};
-  if (i < vh) {
+  if (c.valid) {
     cplx<RT<Pb>> bb[12];
-    SpinorIO<Pb>::load(b, i, bb);
+    SpinorIO<Pb>::load(view<Pb>(b, c), c.t, bb);
     cplx<RT<P>> rr[12], z[12];
     cvt<P, Pb>(bb, rr);
+    zero12<P>(z);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      z[k] = mk<RT<P>>(0, 0);
-      acc[0] += (double)bb[k].re * bb[k].re + (double)bb[k].im * bb[k].im;
-    }
-    SpinorIO<P>::store(r, i, rr);
-    SpinorIO<P>::store(rt, i, rr);
-    SpinorIO<P>::store(p, i, z);
-    SpinorIO<P>::store(v, i, z);
+    for (int k = 0; k < 12; ++k) acc[0] += (double)bb[k].re * bb[k].re + (double)bb[k].im * bb[k].im;
+    SpinorIO<P>::store(view<P>(r, c), c.t, rr);
+    SpinorIO<P>::store(view<P>(rt, c), c.t, rr);
+    SpinorIO<P>::store(view<P>(p, c), c.t, z);
+    SpinorIO<P>::store(view<P>(v, c), c.t, z);
     cplx<RT<Px>> zx[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) zx[k] = mk<RT<Px>>(0, 0);
-    SpinorIO<Px>::store(x, i, zx);
+    zero12<Px>(zx);
+    SpinorIO<Px>::store(view<Px>(x, c), c.t, zx);
   }
-  block_partial<1>(acc, partials, blockIdx.x);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
 // p = r + beta (p - omega v)
 template <int P>
-__global__ void k_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int vh, const SysState *st) {
-  if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
+__global__ void k_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.active) return;
   typedef RT<P> T;
-  const cplx<T> beta = cs<T>(st->beta), omega = cs<T>(st->omega);
+  const cplx<T> beta = cs<T>(st[c.sys].beta), omega = cs<T>(st[c.sys].omega);
   cplx<T> pp[12], rr[12], vv[12];
-  SpinorIO<P>::load(p, i, pp);
-  SpinorIO<P>::load(r, i, rr);
-  SpinorIO<P>::load(v, i, vv);
+  SpinorIO<P>::load(view<P>(p, c), c.t, pp);
+  SpinorIO<P>::load(view<P>(r, c), c.t, rr);
+  SpinorIO<P>::load(view<P>(v, c), c.t, vv);
 #pragma unroll
   for (int k = 0; k < 12; ++k) pp[k] = rr[k] + beta * (pp[k] - omega * vv[k]);
-  SpinorIO<P>::store(p, i, pp);
+  SpinorIO<P>::store(view<P>(p, c), c.t, pp);
 }
 
 // s = r - alpha v
 template <int P>
-__global__ void k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, int vh, const SysState *st) {
-  if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
+__global__ void k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.active) return;
   typedef RT<P> T;
-  const cplx<T> alpha = cs<T>(st->alpha);
+  const cplx<T> alpha = cs<T>(st[c.sys].alpha);
   cplx<T> rr[12], vv[12];
-  SpinorIO<P>::load(r, i, rr);
-  SpinorIO<P>::load(v, i, vv);
+  SpinorIO<P>::load(view<P>(r, c), c.t, rr);
+  SpinorIO<P>::load(view<P>(v, c), c.t, vv);
 #pragma unroll
   for (int k = 0; k < 12; ++k) rr[k] = rr[k] - alpha * vv[k];
-  SpinorIO<P>::store(s, i, rr);
+  SpinorIO<P>::store(view<P>(s, c), c.t, rr);
 }
 
-// x += alpha P + omega S ; r = s - omega t ; <r~, r>, |r|^2 -> rho, beta
+// x += alpha P + omega S ; r = s - omega t ; <r~, r>, |r|^2 per system
 // (P, S) = (p, s) for the plain method, (K p, K s) with a right preconditioner K
 template <int Px, int P, int PC>
 __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s,
-                            SpinorPtr t, int vh, SysState *st, double *partials) {
-  if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                            SpinorPtr t, const SegDesc *segs, int nseg, SysState *st, double *partials) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.sd) return;
   double acc[3] = {0, 0, 0};
-  if (i < vh) {
+  if (c.active) {
     typedef RT<P> T;
     typedef RT<Px> TX;
     cplx<T> pp[12], ss[12], tt[12];
-    SpinorIO<P>::load(p, i, pp);
-    SpinorIO<P>::load(s, i, ss);
-    SpinorIO<P>::load(t, i, tt);
+    SpinorIO<P>::load(view<P>(p, c), c.t, pp);
+    SpinorIO<P>::load(view<P>(s, c), c.t, ss);
+    SpinorIO<P>::load(view<P>(t, c), c.t, tt);
     cplx<TX> xx[12];
-    SpinorIO<Px>::load(x, i, xx);
-    const cplx<TX> al = cs<TX>(st->alpha), omx = cs<TX>(st->omega);
+    SpinorIO<Px>::load(view<Px>(x, c), c.t, xx);
+    const cplx<TX> al = cs<TX>(st[c.sys].alpha), omx = cs<TX>(st[c.sys].omega);
     if (PC) {
       cplx<T> sh[12];
-      SpinorIO<P>::load(sx, i, sh);
+      SpinorIO<P>::load(view<P>(sx, c), c.t, sh);
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
         cplx<TX> pk = mk<TX>((TX)pp[k].re, (TX)pp[k].im), sk = mk<TX>((TX)sh[k].re, (TX)sh[k].im);
@@ -168,10 +203,10 @@ __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p,
         xx[k] = xx[k] + al * pk + omx * sk;
       }
     }
-    SpinorIO<Px>::store(x, i, xx);
-    const cplx<T> om = cs<T>(st->omega);
+    SpinorIO<Px>::store(view<Px>(x, c), c.t, xx);
+    const cplx<T> om = cs<T>(st[c.sys].omega);
     cplx<T> rtt[12];
-    SpinorIO<P>::load(rt, i, rtt);
+    SpinorIO<P>::load(view<P>(rt, c), c.t, rtt);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       ss[k] = ss[k] - om * tt[k];
@@ -179,137 +214,165 @@ __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p,
       acc[1] += (double)rtt[k].re * ss[k].im - (double)rtt[k].im * ss[k].re;
       acc[2] += (double)ss[k].re * ss[k].re + (double)ss[k].im * ss[k].im;
     }
-    SpinorIO<P>::store(r, i, ss);
+    SpinorIO<P>::store(view<P>(r, c), c.t, ss);
   }
-  block_partial<3>(acc, partials, blockIdx.x);
+  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
-// <t, s>, |t|^2 -> omega
-template <int P>
-__global__ void k_omega_dot(SpinorPtr t, SpinorPtr s, int vh, SysState *st, double *partials) {
-  if (st->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// two-vector reductions: <a, b> with |a|^2 (omega of the preconditioned method)
+// or with |b|^2 (rho after a reliable update)
+template <int P, int NORM_OF_A>
+__global__ void k_dot2(SpinorPtr a_, SpinorPtr b_, const SegDesc *segs, int nseg, const SysState *st,
+                       double *partials) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.sd) return;
   double acc[3] = {0, 0, 0};
-  if (i < vh) {
+  if (c.active) {
     cplx<RT<P>> a[12], b[12];
-    SpinorIO<P>::load(t, i, a);
-    SpinorIO<P>::load(s, i, b);
+    SpinorIO<P>::load(view<P>(a_, c), c.t, a);
+    SpinorIO<P>::load(view<P>(b_, c), c.t, b);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       acc[0] += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
       acc[1] += (double)a[k].re * b[k].im - (double)a[k].im * b[k].re;
-      acc[2] += (double)a[k].re * a[k].re + (double)a[k].im * a[k].im;
+      acc[2] += NORM_OF_A ? (double)a[k].re * a[k].re + (double)a[k].im * a[k].im
+                          : (double)b[k].re * b[k].re + (double)b[k].im * b[k].im;
     }
   }
-  block_partial<3>(acc, partials, blockIdx.x);
+  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
-// y = a x + b y
+// y = a x + b y (all systems)
 template <int Px, int Py>
-__global__ void k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, int vh) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
+__global__ void k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, const SegDesc *segs, int nseg) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.valid) return;
   typedef RT<Py> T;
   cplx<RT<Px>> xx[12];
   cplx<T> yy[12];
-  if (a != 0.0) {
-    SpinorIO<Px>::load(x, i, xx);
-  } else {
+  if (a != 0.0) SpinorIO<Px>::load(view<Px>(x, c), c.t, xx);
+  else zero12<Px>(xx);
+  if (b != 0.0) SpinorIO<Py>::load(view<Py>(y, c), c.t, yy);
+  else zero12<Py>(yy);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) xx[k] = mk<RT<Px>>(0, 0);
-  }
-  if (b != 0.0) {
-    SpinorIO<Py>::load(y, i, yy);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 12; ++k) yy[k] = mk<T>(0, 0);
-  }
-#pragma unroll
-  for (int k = 0; k < 12; ++k) {
-    yy[k] = mk<T>((T)(a * xx[k].re + b * yy[k].re), (T)(a * xx[k].im + b * yy[k].im));
-  }
-  SpinorIO<Py>::store(y, i, yy);
+  for (int k = 0; k < 12; ++k) yy[k] = mk<T>((T)(a * xx[k].re + b * yy[k].re), (T)(a * xx[k].im + b * yy[k].im));
+  SpinorIO<Py>::store(view<Py>(y, c), c.t, yy);
 }
 
-// r = b - r, |r|^2 reduced; r in precision Pr, b in Pb
-template <int Pb, int Pr>
-__global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, int vh, double *partials) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// r = b - r (FP64), |r|^2 per system
+__global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, const SegDesc *segs, int nseg, double *partials) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.sd) return;
   double acc[1] = {0};
-  if (i < vh) {
-    typedef RT<Pr> T;
-    cplx<RT<Pb>> bb[12];
-    cplx<T> rr[12];
-    SpinorIO<Pb>::load(b, i, bb);
-    SpinorIO<Pr>::load(r, i, rr);
+  if (c.valid) {
+    cplx<double> bb[12], rr[12];
+    SpinorIO<P64>::load(view<P64>(b, c), c.t, bb);
+    SpinorIO<P64>::load(view<P64>(r, c), c.t, rr);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      rr[k] = mk<T>((T)bb[k].re - rr[k].re, (T)bb[k].im - rr[k].im);
-      acc[0] += (double)rr[k].re * rr[k].re + (double)rr[k].im * rr[k].im;
+      rr[k] = bb[k] - rr[k];
+      acc[0] += norm2(rr[k]);
     }
-    SpinorIO<Pr>::store(r, i, rr);
+    SpinorIO<P64>::store(view<P64>(r, c), c.t, rr);
   }
-  block_partial<1>(acc, partials, blockIdx.x);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
 template <int P>
-__global__ void k_norm2(SpinorPtr x, int vh, double *partials) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_norm2(SpinorPtr x, const SegDesc *segs, int nseg, double *partials) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.sd) return;
   double acc[1] = {0};
-  if (i < vh) {
+  if (c.valid) {
     cplx<RT<P>> xx[12];
-    SpinorIO<P>::load(x, i, xx);
+    SpinorIO<P>::load(view<P>(x, c), c.t, xx);
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[0] += (double)xx[k].re * xx[k].re + (double)xx[k].im * xx[k].im;
   }
-  block_partial<1>(acc, partials, blockIdx.x);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
-// <r~, r>, |r|^2 after a reliable update
-template <int P>
-__global__ void k_reliable_dot(SpinorPtr rt, SpinorPtr r, int vh, double *partials) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[3] = {0, 0, 0};
-  if (i < vh) {
-    cplx<RT<P>> a[12], b[12];
-    SpinorIO<P>::load(rt, i, a);
-    SpinorIO<P>::load(r, i, b);
+// out = C (x + a y) in FP64; a = kappa of the segment when a_is_kappa;
+// which: 0 identity, 1 inverse clover of the parity, 2 clover of the parity
+__global__ void k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, int a_is_kappa, SpinorPtr y, int which, int parity,
+                          const SegDesc *segs, int nseg) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.valid) return;
+  cplx<double> xx[12];
+  SpinorIO<P64>::load(view<P64>(x, c), c.t, xx);
+  const double aa = a_is_kappa ? c.sd->kappa : a;
+  if (aa != 0.0) {
+    cplx<double> yy[12];
+    SpinorIO<P64>::load(view<P64>(y, c), c.t, yy);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) xx[k] = xx[k] + aa * yy[k];
+  }
+  CloverPtr C{};
+  if (which == 1) C = c.sd->Cinv[P64][parity];
+  if (which == 2) C = c.sd->A[parity];
+  if (C.p) {
+    const int site = c.t / c.nrhs;
+    double cb[36];
+    clover_load_block(C, site, 0, cb);
+    clover_apply_block(cb, xx, 0);
+    clover_load_block(C, site, 1, cb);
+    clover_apply_block(cb, xx, 1);
+  }
+  SpinorIO<P64>::store(view<P64>(out, c), c.t, xx);
+}
+
+// |b - (A x - kappa t)|^2 and |b|^2 per system on one parity
+__global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, const SegDesc *segs, int nseg,
+                           double *partials) {
+  const TCtx c = tctx(segs, nseg, nullptr);
+  if (!c.sd) return;
+  double acc[2] = {0, 0};
+  if (c.valid) {
+    cplx<double> bb[12], xx[12], tt[12];
+    SpinorIO<P64>::load(view<P64>(b, c), c.t, bb);
+    SpinorIO<P64>::load(view<P64>(x, c), c.t, xx);
+    SpinorIO<P64>::load(view<P64>(t, c), c.t, tt);
+    const CloverPtr A = c.sd->A[parity];
+    if (A.p) {
+      const int site = c.t / c.nrhs;
+      double cb[36];
+      clover_load_block(A, site, 0, cb);
+      clover_apply_block(cb, xx, 0);
+      clover_load_block(A, site, 1, cb);
+      clover_apply_block(cb, xx, 1);
+    }
+    const double kappa = c.sd->kappa;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      acc[0] += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
-      acc[1] += (double)a[k].re * b[k].im - (double)a[k].im * b[k].re;
-      acc[2] += (double)b[k].re * b[k].re + (double)b[k].im * b[k].im;
+      cplx<double> r = bb[k] - (xx[k] - kappa * tt[k]);
+      acc[0] += norm2(r);
+      acc[1] += norm2(bb[k]);
     }
   }
-  block_partial<3>(acc, partials, blockIdx.x);
+  block_partial_rhs<2>(acc, partials, blockIdx.x, c.t0, c.nrhs);
 }
 
-// out = C (x + a y), C the packed clover (inverse) of the parity or identity
-__global__ void k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, CloverPtr C, int vh) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
-  cplx<double> xx[12];
-  SpinorIO<P64>::load(x, i, xx);
-  if (a != 0.0) {
-    cplx<double> yy[12];
-    SpinorIO<P64>::load(y, i, yy);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) xx[k] = xx[k] + a * yy[k];
-  }
-  if (C.p) {
-    double c[72];
-    CloverIO<P64>::load(C, i, c);
-    clover_apply_site(c, xx);
-  }
-  SpinorIO<P64>::store(out, i, xx);
-}
-
-// ------------------------------------------------------------------ finishing kernel
-// Sums the block partials in a fixed order and applies the epilogue.
+// ---------------------------------------------------------------- finishing kernel
+// One block per system: fixed-order sum of the system's block partials, then
+// the epilogue.
 template <int NR>
-__global__ void __launch_bounds__(256) k_finish(const double *partials, int nblocks, int kind, SysState *st,
-                                                double *out) {
-  if (st && st->done && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO)) return;
+__global__ void __launch_bounds__(256) k_finish(const double *partials, const SegDesc *segs, int nseg, int bsz,
+                                                int kind, SysState *st, double *out) {
+  const int sys = blockIdx.x;
+  int seg = -1, r = 0, blk0 = 0, nb = 0, acc_b = 0;
+  for (int k = 0; k < nseg; ++k) {
+    const int n = (segs[k].nthr + bsz - 1) / bsz;
+    if (sys >= segs[k].sys0 && sys < segs[k].sys0 + segs[k].nrhs) {
+      seg = k;
+      r = sys - segs[k].sys0;
+      blk0 = acc_b;
+      nb = n;
+      break;
+    }
+    acc_b += n;
+  }
+  if (seg < 0) return;
+  if (st && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO || kind == FK_RELIABLE) && st[sys].done) return;
   double acc4[4][NR];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -317,15 +380,15 @@ __global__ void __launch_bounds__(256) k_finish(const double *partials, int nblo
     for (int k = 0; k < NR; ++k) acc4[u][k] = 0;
   const int bs = blockDim.x;
   int b = threadIdx.x;
-  for (; b + 3 * bs < nblocks; b += 4 * bs) {
+  for (; b + 3 * bs < nb; b += 4 * bs) {
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int k = 0; k < NR; ++k) acc4[u][k] += partials[(size_t)(b + u * bs) * NR + k];
+      for (int k = 0; k < NR; ++k) acc4[u][k] += partials[((size_t)(blk0 + b + u * bs) * kMaxRhs + r) * NR + k];
   }
-  for (; b < nblocks; b += bs) {
+  for (; b < nb; b += bs) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) acc4[0][k] += partials[(size_t)b * NR + k];
+    for (int k = 0; k < NR; ++k) acc4[0][k] += partials[((size_t)(blk0 + b) * kMaxRhs + r) * NR + k];
   }
   double acc[NR];
 #pragma unroll
@@ -337,130 +400,147 @@ __global__ void __launch_bounds__(256) k_finish(const double *partials, int nblo
   double tot[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < NR; ++k) tot[k] = tot_sh[k];
+  SysState *s = st ? st + sys : nullptr;
   switch (kind) {
-    case FK_INIT: fin_norm(st, tot[0], FIN_INIT, nullptr); break;
+    case FK_INIT: fin_init(s, tot[0]); break;
     case FK_STORE:
-      for (int k = 0; k < NR; ++k) out[k] = tot[k];
+      for (int k = 0; k < NR; ++k) out[(size_t)sys * NR + k] = tot[k];
       break;
     case FK_ACCUM:
-      for (int k = 0; k < NR; ++k) out[k] += tot[k];
+      for (int k = 0; k < NR; ++k) out[(size_t)sys * NR + k] += tot[k];
       break;
-    case FK_ALPHA: fin_alpha(st, tot[0], tot[1]); break;
-    case FK_OMEGA: fin_omega(st, tot[0], tot[1], tot[2]); break;
-    case FK_RHO: fin_rho(st, tot[0], tot[1], tot[2]); break;
-    case FK_RELIABLE: fin_reliable(st, tot[0], tot[1], tot[2]); break;
+    case FK_STORE_TMP:
+      for (int k = 0; k < NR; ++k) s->tmp[k] = tot[k];
+      break;
+    case FK_ALPHA: fin_alpha(s, tot[0], tot[1]); break;
+    case FK_OMEGA: fin_omega(s, tot[0], tot[1], tot[2]); break;
+    case FK_RHO: fin_rho(s, tot[0], tot[1], tot[2]); break;
+    case FK_RELIABLE: fin_reliable(s, tot[0], tot[1], tot[2]); break;
     default: break;
   }
 }
 
-int finish(const double *partials, int nblocks, int nr, int kind, SysState *st, double *out, hipStream_t s) {
+int finish(const BatchOp &op, const double *partials, int bsz, int nr, int kind, SysState *st, double *out,
+           hipStream_t s) {
   if (kind == FK_NONE) return 0;
+  const int nsys = op.nsys;
+  const SegDesc *d = op.dev_segs;
+  const int nseg = (int)op.segs.size();
   switch (nr) {
-    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
-    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
-    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(256), 0, s, partials, nblocks, kind, st, out); break;
+    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
+    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
+    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
     default: set_error(QCU_ERR_ARG, "finish: bad reduction width"); return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
-// ------------------------------------------------------------------ launchers
-static inline int nblk(int vh, int bs) { return (vh + bs - 1) / bs; }
-
-int bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, int prec, SpinorPtr b, int pb, int vh,
-              SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s) {
-  const int bs = cfg.block, nb = nblk(vh, bs);
-  // x precision: FP64 for the plain solver, FP32 for the sloppy accumulator
-  if (pb == P64 && prec == P64)
-    hipLaunchKernelGGL((k_bicg_init<P64, P64, P64>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
-  else if (pb == P64 && prec == P32)
-    hipLaunchKernelGGL((k_bicg_init<P64, P32, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
-  else if (pb == P64 && prec == P16)
-    hipLaunchKernelGGL((k_bicg_init<P64, P16, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, vh, partials);
-  else {
-    set_error(QCU_ERR_ARG, "bicg_init: unsupported precision pair");
-    return QCU_ERR_ARG;
-  }
-  QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 1, FK_INIT, st, nullptr, s);
+// ---------------------------------------------------------------- launchers
+int batch_blocks(const BatchOp &op, int bsz) {
+  int nb = 0;
+  for (const SegDesc &d : op.segs) nb += (d.nthr + bsz - 1) / bsz;
+  return nb;
 }
 
-int bicg_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st, const LaunchCfg &cfg,
-                  hipStream_t s) {
-  const int bs = cfg.block, nb = nblk(vh, bs);
+#define QCU_SEGS op.dev_segs, (int)op.segs.size()
+
+int bicg_init(const BatchOp &op, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, int prec,
+              SpinorPtr b, SysState *st, double *partials, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  // x precision: FP64 for the plain solver, FP32 for the sloppy accumulator
+  if (prec == P64)
+    hipLaunchKernelGGL((k_bicg_init<P64, P64, P64>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, QCU_SEGS, partials);
+  else if (prec == P32)
+    hipLaunchKernelGGL((k_bicg_init<P64, P32, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, QCU_SEGS, partials);
+  else
+    hipLaunchKernelGGL((k_bicg_init<P64, P16, P32>), dim3(nb), dim3(bs), 0, s, r, rt, p, v, x, b, QCU_SEGS, partials);
+  QCU_HIP(hipGetLastError());
+  return finish(op, partials, bs, 1, FK_INIT, st, nullptr, s);
+}
+
+int bicg_p_update(const BatchOp &op, SpinorPtr p, SpinorPtr r, SpinorPtr v, int prec, SysState *st, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_p_update<P64>, dim3(nb), dim3(bs), 0, s, p, r, v, vh, st); break;
-    case P32: hipLaunchKernelGGL(k_p_update<P32>, dim3(nb), dim3(bs), 0, s, p, r, v, vh, st); break;
-    default: hipLaunchKernelGGL(k_p_update<P16>, dim3(nb), dim3(bs), 0, s, p, r, v, vh, st); break;
+    case P64: hipLaunchKernelGGL(k_p_update<P64>, dim3(nb), dim3(bs), 0, s, p, r, v, QCU_SEGS, st); break;
+    case P32: hipLaunchKernelGGL(k_p_update<P32>, dim3(nb), dim3(bs), 0, s, p, r, v, QCU_SEGS, st); break;
+    default: hipLaunchKernelGGL(k_p_update<P16>, dim3(nb), dim3(bs), 0, s, p, r, v, QCU_SEGS, st); break;
   }
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
-int bicg_s_update(SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, int vh, SysState *st, const LaunchCfg &cfg,
-                  hipStream_t s) {
-  const int bs = cfg.block, nb = nblk(vh, bs);
+int bicg_s_update(const BatchOp &op, SpinorPtr s_, SpinorPtr r, SpinorPtr v, int prec, SysState *st, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_s_update<P64>, dim3(nb), dim3(bs), 0, s, s_, r, v, vh, st); break;
-    case P32: hipLaunchKernelGGL(k_s_update<P32>, dim3(nb), dim3(bs), 0, s, s_, r, v, vh, st); break;
-    default: hipLaunchKernelGGL(k_s_update<P16>, dim3(nb), dim3(bs), 0, s, s_, r, v, vh, st); break;
+    case P64: hipLaunchKernelGGL(k_s_update<P64>, dim3(nb), dim3(bs), 0, s, s_, r, v, QCU_SEGS, st); break;
+    case P32: hipLaunchKernelGGL(k_s_update<P32>, dim3(nb), dim3(bs), 0, s, s_, r, v, QCU_SEGS, st); break;
+    default: hipLaunchKernelGGL(k_s_update<P16>, dim3(nb), dim3(bs), 0, s, s_, r, v, QCU_SEGS, st); break;
   }
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
 template <int PC>
-static int xr_launch(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s_,
-                     SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
-                     hipStream_t s) {
-  const int bs = cfg.block, nb = nblk(vh, bs);
+static int xr_launch(const BatchOp &op, SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx,
+                     SpinorPtr s_, SpinorPtr t, int prec, SysState *st, double *partials, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
   if (px == P64 && prec == P64)
-    hipLaunchKernelGGL((k_xr_update<P64, P64, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
+    hipLaunchKernelGGL((k_xr_update<P64, P64, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, QCU_SEGS, st,
                        partials);
   else if (px == P32 && prec == P32)
-    hipLaunchKernelGGL((k_xr_update<P32, P32, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
+    hipLaunchKernelGGL((k_xr_update<P32, P32, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, QCU_SEGS, st,
                        partials);
   else if (px == P32 && prec == P16)
-    hipLaunchKernelGGL((k_xr_update<P32, P16, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, vh, st,
+    hipLaunchKernelGGL((k_xr_update<P32, P16, PC>), dim3(nb), dim3(bs), 0, s, x, r, rt, p, sx, s_, t, QCU_SEGS, st,
                        partials);
   else {
     set_error(QCU_ERR_ARG, "bicg_xr_update: unsupported precision pair");
     return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 3, FK_RHO, st, nullptr, s);
+  return finish(op, partials, bs, 3, FK_RHO, st, nullptr, s);
 }
 
-int bicg_xr_update(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr s_, SpinorPtr t, int prec,
-                   int vh, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s) {
-  return xr_launch<0>(x, px, r, rt, p, s_, s_, t, prec, vh, st, partials, cfg, s);
+int bicg_xr_update(const BatchOp &op, SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr s_,
+                   SpinorPtr t, int prec, SysState *st, double *partials, hipStream_t s) {
+  return xr_launch<0>(op, x, px, r, rt, p, s_, s_, t, prec, st, partials, s);
 }
 
-int bicg_xr_update_pc(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr ph, SpinorPtr sh, SpinorPtr s_,
-                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
-                      hipStream_t s) {
-  return xr_launch<1>(x, px, r, rt, ph, sh, s_, t, prec, vh, st, partials, cfg, s);
+int bicg_xr_update_pc(const BatchOp &op, SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr ph, SpinorPtr sh,
+                      SpinorPtr s_, SpinorPtr t, int prec, SysState *st, double *partials, hipStream_t s) {
+  return xr_launch<1>(op, x, px, r, rt, ph, sh, s_, t, prec, st, partials, s);
 }
 
-int omega_dot(SpinorPtr t, SpinorPtr s_, int prec, int vh, SysState *st, double *partials, hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
+template <int NA>
+static int dot2(const BatchOp &op, SpinorPtr a, SpinorPtr b, int prec, SysState *st, double *partials, int kind,
+                hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_omega_dot<P64>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
-    case P32: hipLaunchKernelGGL(k_omega_dot<P32>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
-    default: hipLaunchKernelGGL(k_omega_dot<P16>, dim3(nb), dim3(bs), 0, s, t, s_, vh, st, partials); break;
+    case P64: hipLaunchKernelGGL((k_dot2<P64, NA>), dim3(nb), dim3(bs), 0, s, a, b, QCU_SEGS, st, partials); break;
+    case P32: hipLaunchKernelGGL((k_dot2<P32, NA>), dim3(nb), dim3(bs), 0, s, a, b, QCU_SEGS, st, partials); break;
+    default: hipLaunchKernelGGL((k_dot2<P16, NA>), dim3(nb), dim3(bs), 0, s, a, b, QCU_SEGS, st, partials); break;
   }
   QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 3, FK_OMEGA, st, nullptr, s);
+  return finish(op, partials, bs, 3, kind, st, nullptr, s);
 }
 
-int axpby(double a, SpinorPtr x, int px, double b, SpinorPtr y, int py, int vh, hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
-#define QCU_AXPBY(A, B)                                                                       \
-  if (px == A && py == B) {                                                                 \
-    hipLaunchKernelGGL((k_axpby<A, B>), dim3(nb), dim3(bs), 0, s, a, x, b, y, vh);          \
-    QCU_HIP(hipGetLastError());                                                             \
-    return 0;                                                                               \
+int omega_dot(const BatchOp &op, SpinorPtr t, SpinorPtr s_, int prec, SysState *st, double *partials, hipStream_t s) {
+  return dot2<1>(op, t, s_, prec, st, partials, FK_OMEGA, s);
+}
+
+int reliable_dot(const BatchOp &op, SpinorPtr rt, SpinorPtr r, int prec, SysState *st, double *partials,
+                 hipStream_t s) {
+  return dot2<0>(op, rt, r, prec, st, partials, FK_RELIABLE, s);
+}
+
+int axpby(const BatchOp &op, double a, SpinorPtr x, int px, double b, SpinorPtr y, int py, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+#define QCU_AXPBY(A, B)                                                                         \
+  if (px == A && py == B) {                                                                   \
+    hipLaunchKernelGGL((k_axpby<A, B>), dim3(nb), dim3(bs), 0, s, a, x, b, y, QCU_SEGS);      \
+    QCU_HIP(hipGetLastError());                                                               \
+    return 0;                                                                                 \
   }
   QCU_AXPBY(P64, P64)
   QCU_AXPBY(P32, P64)
@@ -476,48 +556,47 @@ int axpby(double a, SpinorPtr x, int px, double b, SpinorPtr y, int py, int vh, 
   return QCU_ERR_ARG;
 }
 
-int spinor_convert(SpinorPtr dst, int pdst, SpinorPtr src, int psrc, int vh, hipStream_t s) {
-  return axpby(1.0, src, psrc, 0.0, dst, pdst, vh, s);
+int spinor_convert(const BatchOp &op, SpinorPtr dst, int pdst, SpinorPtr src, int psrc, hipStream_t s) {
+  return axpby(op, 1.0, src, psrc, 0.0, dst, pdst, s);
 }
 
-int zero_spinor(SpinorPtr x, int px, int vh, hipStream_t s) { return axpby(0.0, x, px, 0.0, x, px, vh, s); }
+int zero_spinor(const BatchOp &op, SpinorPtr x, int px, hipStream_t s) { return axpby(op, 0.0, x, px, 0.0, x, px, s); }
 
-int xmy_norm(SpinorPtr b, int pb, SpinorPtr r, int pr, int vh, SysState *st, double *partials, int fin,
+int xmy_norm(const BatchOp &op, SpinorPtr b, SpinorPtr r, SysState *st, double *partials, int store,
              hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
-  QCU_CHECK(pb == P64 && pr == P64, QCU_ERR_ARG, "xmy_norm: unsupported precision pair");
-  hipLaunchKernelGGL((k_xmy_norm<P64, P64>), dim3(nb), dim3(bs), 0, s, b, r, vh, partials);
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_xmy_norm, dim3(nb), dim3(bs), 0, s, b, r, QCU_SEGS, partials);
   QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 1, fin == FIN_STORE ? FK_STORE : FK_NONE, st, st ? st->tmp : nullptr, s);
+  return finish(op, partials, bs, 1, store ? FK_STORE_TMP : FK_NONE, st, nullptr, s);
 }
 
-int reliable_dot(SpinorPtr rt, SpinorPtr r, int prec, int vh, SysState *st, double *partials, hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
-  switch (prec) {
-    case P64: hipLaunchKernelGGL(k_reliable_dot<P64>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
-    case P32: hipLaunchKernelGGL(k_reliable_dot<P32>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
-    default: hipLaunchKernelGGL(k_reliable_dot<P16>, dim3(nb), dim3(bs), 0, s, rt, r, vh, partials); break;
+int norm2_dev(const BatchOp &op, SpinorPtr x, int px, double *out_dev, double *partials, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  switch (px) {
+    case P64: hipLaunchKernelGGL(k_norm2<P64>, dim3(nb), dim3(bs), 0, s, x, QCU_SEGS, partials); break;
+    case P32: hipLaunchKernelGGL(k_norm2<P32>, dim3(nb), dim3(bs), 0, s, x, QCU_SEGS, partials); break;
+    default: hipLaunchKernelGGL(k_norm2<P16>, dim3(nb), dim3(bs), 0, s, x, QCU_SEGS, partials); break;
   }
   QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 3, FK_RELIABLE, st, nullptr, s);
+  return finish(op, partials, bs, 1, FK_STORE, nullptr, out_dev, s);
 }
 
-int cl_axpy(SpinorPtr out, SpinorPtr x, double a, SpinorPtr y, CloverPtr C, int vh, hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
-  hipLaunchKernelGGL(k_cl_axpy, dim3(nb), dim3(bs), 0, s, out, x, a, y, C, vh);
+int cl_axpy(const BatchOp &op, SpinorPtr out, SpinorPtr x, double a, bool a_is_kappa, SpinorPtr y, int which,
+            int parity, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_cl_axpy, dim3(nb), dim3(bs), 0, s, out, x, a, (int)a_is_kappa, y, which, parity, QCU_SEGS);
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
-int norm2_dev(SpinorPtr x, int px, int vh, double *out_dev, double *partials, hipStream_t s) {
-  const int bs = 256, nb = nblk(vh, bs);
-  switch (px) {
-    case P64: hipLaunchKernelGGL(k_norm2<P64>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
-    case P32: hipLaunchKernelGGL(k_norm2<P32>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
-    default: hipLaunchKernelGGL(k_norm2<P16>, dim3(nb), dim3(bs), 0, s, x, vh, partials); break;
-  }
+int residual_parity(const BatchOp &op, SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, double *partials,
+                    double *out, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(bs), 0, s, b, x, t, parity, QCU_SEGS, partials);
   QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 1, FK_STORE, nullptr, out_dev, s);
+  return finish(op, partials, bs, 2, FK_ACCUM, nullptr, out, s);
 }
+
+#undef QCU_SEGS
 
 }  // namespace qcu

@@ -10,7 +10,6 @@
 // gauge field instead of on every call.
 #include "qcu_internal.h"
 #include "qcu_reduce.h"
-#include "qcu_solver.h"
 
 namespace qcu {
 
@@ -281,40 +280,6 @@ int clover_convert(const void *c64, const Geom &g, int prec, void *dst, float *n
   hipLaunchKernelGGL(k_clover_cvt, dim3(nb), dim3(bs), 0, s, (const double2 *)c64, g.vh, prec, dst, norm);
   QCU_HIP(hipGetLastError());
   return 0;
-}
-
-// r = b - (A x - kappa t) on one parity (A packed, or identity when null);
-// |r|^2 and |b|^2 accumulated into out[0], out[1]
-__global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, double kappa, int vh,
-                           double *partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[2] = {0, 0};
-  if (i < vh) {
-    cd bb[12], xx[12], tt[12];
-    SpinorIO<P64>::load(b, i, bb);
-    SpinorIO<P64>::load(x, i, xx);
-    SpinorIO<P64>::load(t, i, tt);
-    if (A.p) {
-      double c[72];
-      CloverIO<P64>::load(A, i, c);
-      clover_apply_site(c, xx);
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      cd r = bb[k] - (xx[k] - kappa * tt[k]);
-      acc[0] += norm2(r);
-      acc[1] += norm2(bb[k]);
-    }
-  }
-  block_partial<2>(acc, partials, blockIdx.x);
-}
-
-int residual_parity(SpinorPtr b, SpinorPtr x, SpinorPtr t, CloverPtr A, double kappa, int vh, double *partials,
-                    double *out, hipStream_t s) {
-  const int bs = 256, nb = (vh + bs - 1) / bs;
-  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(bs), 0, s, b, x, t, A, kappa, vh, partials);
-  QCU_HIP(hipGetLastError());
-  return finish(partials, nb, 2, FK_ACCUM, nullptr, out, s);
 }
 
 }  // namespace qcu

@@ -15,6 +15,7 @@
 // (src/bistabcg.cu:85-96), and fused FP64 grid reductions feeding the BiCGSTAB
 // scalar epilogues.  The stencil is HBM bound (about 0.5 flop per byte), so
 // no matrix cores are involved.
+#include "qcu_batch.h"
 #include "qcu_internal.h"
 #include "qcu_reduce.h"
 #include "qcu_state.h"
@@ -112,14 +113,12 @@ QCU_D void hop_dir(cplx<typename real_of<P>::type> (&d)[12], const LinkPtr &U, i
 }
 
 struct HopKArgs {
-  SpinorPtr out, in, x, y;
-  LinkPtr U;
-  CloverPtr C;
-  Geom g;
+  SpinorPtr out, in, x, y;  // batch vectors
+  const SegDesc *segs;
+  int nseg;
   int parity;
   int remap;
-  double k2;
-  SysState *st;
+  SysState *st;  // per-system states, or null
   double *partials;
 };
 
@@ -127,54 +126,63 @@ struct HopKArgs {
 // directions (most memory-level parallelism, most registers); 1 and 2 fence the
 // instruction schedule after every direction / every pair of directions, which
 // caps the live registers and raises occupancy (the FP64 kernel needs this).
-#define QCU_GROUP_FENCE(G, k)                                  \
-  do {                                                         \
-    if ((G) == 1 || ((G) == 2 && ((k) & 1))) __builtin_amdgcn_sched_barrier(0); \
+#define QCU_GROUP_FENCE(G, k)                                                      \
+  do {                                                                             \
+    if ((G) == 1 || ((G) == 2 && ((k) & 1))) __builtin_amdgcn_sched_barrier(0);    \
   } while (0)
 
 template <int P, int CLOVER, int EPI, int RED, int G>
 __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
   typedef typename real_of<P>::type T;
-  // a solver state, when given, stops the chain of kernels once the solve has stopped
-  if (a.st && a.st->done) return;
   const int blk = remap_block(blockIdx.x, gridDim.x, a.remap);
-  const int i = blk * blockDim.x + threadIdx.x;
-  const Geom &g = a.g;
-  const bool active = i < g.vh;
+  const BlockSeg bs = find_seg(a.segs, a.nseg, blk, blockDim.x);
+  if (bs.seg < 0) return;  // uniform: a block past the last segment
+  const SegDesc &sd = a.segs[bs.seg];
+  const Geom g = sd.g;
+  const int nrhs = sd.nrhs;
+  const int t = bs.t0 + threadIdx.x;
+  const int site = t / nrhs;
+  const int rhs = t - site * nrhs;
+  bool active = t < sd.nthr;
+  // a solver state, when given, stops the work of its system once it has stopped
+  if (a.st && active && a.st[sd.sys0 + rhs].done) active = false;
   cplx<T> d[12];
   double red[3] = {0, 0, 0};
   if (active) {
+    const SpinorPtr in = seg_view(a.in, P, sd.voff, sd.nthr);
+    const LinkPtr U = sd.U[P];
+    const int i = site;
     const Site s = decode(g, i, a.parity);
     const int p = a.parity, q = 1 - p;
     const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
 #pragma unroll
     for (int k = 0; k < 12; ++k) d[k] = mk<T>(0, 0);
-    // x: a site with odd full x has its x-1 neighbour at the same half index
     {
       // branch-free index arithmetic keeps the body one basic block, so the
-      // scheduling fences below bound the live registers
+      // scheduling fences below bound the live registers; a site with odd
+      // full x has its x-1 neighbour at the same half index
       const int xe = 1 - s.xodd;
       const int jb = i - xe + xe * (int)(s.xh == 0) * g.lxh;
       const int jf = i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh;
-      hop_dir<P, 0, 0>(d, a.U, q * 4 + 0, jb, a.in, jb, (T)1);
+      hop_dir<P, 0, 0>(d, U, q * 4 + 0, jb, in, jb * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 0);
-      hop_dir<P, 0, 1>(d, a.U, p * 4 + 0, i, a.in, jf, (T)1);
+      hop_dir<P, 0, 1>(d, U, p * 4 + 0, i, in, jf * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 1);
     }
     {
       const int jb = i - sy + (int)(s.y == 0) * g.ly * sy;
       const int jf = i + sy - (int)(s.y == g.ly - 1) * g.ly * sy;
-      hop_dir<P, 1, 0>(d, a.U, q * 4 + 1, jb, a.in, jb, (T)1);
+      hop_dir<P, 1, 0>(d, U, q * 4 + 1, jb, in, jb * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 2);
-      hop_dir<P, 1, 1>(d, a.U, p * 4 + 1, i, a.in, jf, (T)1);
+      hop_dir<P, 1, 1>(d, U, p * 4 + 1, i, in, jf * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 3);
     }
     {
       const int jb = i - sz + (int)(s.z == 0) * g.lz * sz;
       const int jf = i + sz - (int)(s.z == g.lz - 1) * g.lz * sz;
-      hop_dir<P, 2, 0>(d, a.U, q * 4 + 2, jb, a.in, jb, (T)1);
+      hop_dir<P, 2, 0>(d, U, q * 4 + 2, jb, in, jb * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 4);
-      hop_dir<P, 2, 1>(d, a.U, p * 4 + 2, i, a.in, jf, (T)1);
+      hop_dir<P, 2, 1>(d, U, p * 4 + 2, i, in, jf * nrhs + rhs, (T)1);
       QCU_GROUP_FENCE(G, 5);
     }
     {
@@ -183,30 +191,31 @@ __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
       const int flip = (g.tbc < 0);
       const T phb = (T)(1 - 2 * (flip & (int)(s.t == 0)));
       const T phf = (T)(1 - 2 * (flip & (int)(s.t == g.lt - 1)));
-      hop_dir<P, 3, 0>(d, a.U, q * 4 + 3, jb, a.in, jb, phb);
+      hop_dir<P, 3, 0>(d, U, q * 4 + 3, jb, in, jb * nrhs + rhs, phb);
       QCU_GROUP_FENCE(G, 6);
-      hop_dir<P, 3, 1>(d, a.U, p * 4 + 3, i, a.in, jf, phf);
+      hop_dir<P, 3, 1>(d, U, p * 4 + 3, i, in, jf * nrhs + rhs, phf);
       QCU_GROUP_FENCE(G, 7);
     }
     if (CLOVER) {
+      const CloverPtr C = sd.Cinv[P][p];
       if (P == P16) {
         T c[72];
-        CloverIO<P>::load(a.C, i, c);
+        CloverIO<P>::load(C, i, c);
         clover_apply_site(c, d);
       } else {
         // one chiral block at a time bounds the live registers
         T c[36];
-        clover_load_block(a.C, i, 0, c);
+        clover_load_block(C, i, 0, c);
         clover_apply_block(c, d, 0);
         QCU_GROUP_FENCE(G, 1);
-        clover_load_block(a.C, i, 1, c);
+        clover_load_block(C, i, 1, c);
         clover_apply_block(c, d, 1);
       }
     }
     if (EPI == EPI_XPAY) {
       cplx<T> x[12];
-      SpinorIO<P>::load(a.x, i, x);
-      const T k2 = (T)a.k2;
+      SpinorIO<P>::load(seg_view(a.x, P, sd.voff, sd.nthr), t, x);
+      const T k2 = (T)(sd.kappa * sd.kappa);
 #pragma unroll
       for (int k = 0; k < 12; ++k) d[k] = x[k] - k2 * d[k];
       if (RED == RED_OUT_X) {
@@ -221,26 +230,31 @@ __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
     }
     if (RED == RED_DOT_Y_OUT) {
       cplx<T> y[12];
-      SpinorIO<P>::load(a.y, i, y);
+      SpinorIO<P>::load(seg_view(a.y, P, sd.voff, sd.nthr), t, y);
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
         red[0] += (double)y[k].re * (double)d[k].re + (double)y[k].im * (double)d[k].im;
         red[1] += (double)y[k].re * (double)d[k].im - (double)y[k].im * (double)d[k].re;
       }
     }
-    SpinorIO<P>::store(a.out, i, d);
+    SpinorIO<P>::store(seg_view(a.out, P, sd.voff, sd.nthr), t, d);
   }
   if (RED == RED_DOT_Y_OUT) {
     double v[2] = {red[0], red[1]};
-    block_partial<2>(v, a.partials, blockIdx.x);
+    block_partial_rhs<2>(v, a.partials, blk, bs.t0, nrhs);
   } else if (RED == RED_OUT_X) {
-    block_partial<3>(red, a.partials, blockIdx.x);
+    block_partial_rhs<3>(red, a.partials, blk, bs.t0, nrhs);
   }
 }
 
+static int total_blocks(const std::vector<SegDesc> &segs, int bsz) {
+  int nb = 0;
+  for (const SegDesc &d : segs) nb += (d.nthr + bsz - 1) / bsz;
+  return nb;
+}
+
 template <int P, int CLOVER, int EPI, int RED>
-static int launch(const HopKArgs &a, const LaunchCfg &cfg, hipStream_t s) {
-  int nb = (a.g.vh + cfg.block - 1) / cfg.block;
+static int launch(const HopKArgs &a, int nb, const LaunchCfg &cfg, hipStream_t s) {
   switch (cfg.group) {
     case 1: hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED, 1>), dim3(nb), dim3(cfg.block), 0, s, a); break;
     case 2: hipLaunchKernelGGL((k_hop<P, CLOVER, EPI, RED, 2>), dim3(nb), dim3(cfg.block), 0, s, a); break;
@@ -251,26 +265,25 @@ static int launch(const HopKArgs &a, const LaunchCfg &cfg, hipStream_t s) {
 }
 
 template <int P>
-static int dispatch(const HopKArgs &a, int clover, int epi, int red, const LaunchCfg &cfg,
-                    hipStream_t s) {
+static int dispatch(const HopKArgs &a, int nb, int clover, int epi, int red, const LaunchCfg &cfg, hipStream_t s) {
   if (epi == EPI_PLAIN) {
-    return clover ? launch<P, 1, EPI_PLAIN, RED_NONE>(a, cfg, s)
-                  : launch<P, 0, EPI_PLAIN, RED_NONE>(a, cfg, s);
+    return clover ? launch<P, 1, EPI_PLAIN, RED_NONE>(a, nb, cfg, s)
+                  : launch<P, 0, EPI_PLAIN, RED_NONE>(a, nb, cfg, s);
   }
   if (red == RED_NONE)
-    return clover ? launch<P, 1, EPI_XPAY, RED_NONE>(a, cfg, s)
-                  : launch<P, 0, EPI_XPAY, RED_NONE>(a, cfg, s);
+    return clover ? launch<P, 1, EPI_XPAY, RED_NONE>(a, nb, cfg, s)
+                  : launch<P, 0, EPI_XPAY, RED_NONE>(a, nb, cfg, s);
   if (red == RED_DOT_Y_OUT)
-    return clover ? launch<P, 1, EPI_XPAY, RED_DOT_Y_OUT>(a, cfg, s)
-                  : launch<P, 0, EPI_XPAY, RED_DOT_Y_OUT>(a, cfg, s);
-  return clover ? launch<P, 1, EPI_XPAY, RED_OUT_X>(a, cfg, s)
-                : launch<P, 0, EPI_XPAY, RED_OUT_X>(a, cfg, s);
+    return clover ? launch<P, 1, EPI_XPAY, RED_DOT_Y_OUT>(a, nb, cfg, s)
+                  : launch<P, 0, EPI_XPAY, RED_DOT_Y_OUT>(a, nb, cfg, s);
+  return clover ? launch<P, 1, EPI_XPAY, RED_OUT_X>(a, nb, cfg, s)
+                : launch<P, 0, EPI_XPAY, RED_OUT_X>(a, nb, cfg, s);
 }
 
-int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, SpinorPtr x, double k2,
-        int red, SpinorPtr y, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s,
-        bool use_clover) {
+int hop(const BatchOp &op, int prec, SpinorPtr out, SpinorPtr in, int parity, int epi, SpinorPtr x, int red,
+        SpinorPtr y, SysState *st, double *partials, const LaunchCfg &cfg, hipStream_t s, bool use_clover) {
   QCU_CHECK(parity == 0 || parity == 1, QCU_ERR_ARG, "parity must be 0 or 1");
+  QCU_CHECK(prec >= P64 && prec <= P16, QCU_ERR_ARG, "unknown precision");
   QCU_CHECK(cfg.block >= 64 && cfg.block <= 256 && cfg.block % 64 == 0, QCU_ERR_ARG,
             "block size must be a multiple of 64 in [64, 256]");
   QCU_CHECK(red == RED_NONE || (epi == EPI_XPAY && st && partials), QCU_ERR_ARG,
@@ -280,26 +293,23 @@ int hop(const HopOp &op, SpinorPtr out, SpinorPtr in, int parity, int epi, Spino
   a.in = in;
   a.x = x;
   a.y = y;
-  a.U = op.U;
-  a.C = op.C[parity];
-  a.g = op.g;
+  a.segs = op.dev_segs;
+  a.nseg = (int)op.segs.size();
   a.parity = parity;
   a.remap = cfg.remap;
-  a.k2 = k2;
   a.st = st;
   a.partials = partials;
-  int clover = use_clover && op.C[parity].p != nullptr;
+  const int clover = use_clover && op.has_clover;
+  const int nb = total_blocks(op.segs, cfg.block);
   int rc;
-  switch (op.prec) {
-    case P64: rc = dispatch<P64>(a, clover, epi, red, cfg, s); break;
-    case P32: rc = dispatch<P32>(a, clover, epi, red, cfg, s); break;
-    case P16: rc = dispatch<P16>(a, clover, epi, red, cfg, s); break;
-    default: set_error(QCU_ERR_ARG, "unknown precision"); return QCU_ERR_ARG;
+  switch (prec) {
+    case P64: rc = dispatch<P64>(a, nb, clover, epi, red, cfg, s); break;
+    case P32: rc = dispatch<P32>(a, nb, clover, epi, red, cfg, s); break;
+    default: rc = dispatch<P16>(a, nb, clover, epi, red, cfg, s); break;
   }
   if (rc || red == RED_NONE) return rc;
-  const int nb = (op.g.vh + cfg.block - 1) / cfg.block;
-  return red == RED_DOT_Y_OUT ? finish(partials, nb, 2, FK_ALPHA, st, nullptr, s)
-                              : finish(partials, nb, 3, FK_OMEGA, st, nullptr, s);
+  return red == RED_DOT_Y_OUT ? finish(op, partials, cfg.block, 2, FK_ALPHA, st, nullptr, s)
+                              : finish(op, partials, cfg.block, 3, FK_OMEGA, st, nullptr, s);
 }
 
 }  // namespace qcu

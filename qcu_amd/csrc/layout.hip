@@ -57,8 +57,9 @@ int gauge_ext_to_internal(const void *ext, const Geom &g, int prec, void *a, voi
   return 0;
 }
 
+// external one-parity field [Vh][4][3] <-> system (segment seg, rhs r) of a batch vector
 template <int P>
-__global__ void k_spinor_in(const double2 *ext, int vh, SpinorPtr dst) {
+__global__ void k_spinor_in(const double2 *ext, int vh, int nrhs, int rhs, SpinorPtr dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   typedef typename real_of<P>::type T;
@@ -69,39 +70,50 @@ __global__ void k_spinor_in(const double2 *ext, int vh, SpinorPtr dst) {
     double2 a = src[k];
     v[k] = mk<T>((T)a.x, (T)a.y);
   }
-  SpinorIO<P>::store(dst, i, v);
+  SpinorIO<P>::store(dst, i * nrhs + rhs, v);
 }
 
 template <int P>
-__global__ void k_spinor_out(SpinorPtr src, int vh, double2 *ext) {
+__global__ void k_spinor_out(SpinorPtr src, int vh, int nrhs, int rhs, double2 *ext) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   typedef typename real_of<P>::type T;
   cplx<T> v[12];
-  SpinorIO<P>::load(src, i, v);
+  SpinorIO<P>::load(src, i * nrhs + rhs, v);
   double2 *dst = ext + (size_t)i * 12;
 #pragma unroll
   for (int k = 0; k < 12; ++k) dst[k] = make_double2((double)v[k].re, (double)v[k].im);
 }
 
-int spinor_ext_to_internal(const void *ext, const Geom &g, SpinorPtr dst, int prec, hipStream_t s) {
-  const int bs = 256, nb = (g.vh + bs - 1) / bs;
+int spinor_ext_to_batch(const void *ext, const BatchOp &op, int seg, int rhs, SpinorPtr dst, int prec,
+                        hipStream_t s) {
+  QCU_CHECK(seg >= 0 && seg < (int)op.segs.size(), QCU_ERR_ARG, "bad segment");
+  const SegDesc &d = op.segs[seg];
+  QCU_CHECK(rhs >= 0 && rhs < d.nrhs, QCU_ERR_ARG, "bad right-hand side");
+  const SpinorPtr v = seg_view(dst, prec, d.voff, d.nthr);
+  const int vh = d.g.vh, bs = 256, nb = (vh + bs - 1) / bs;
+  const double2 *e = (const double2 *)ext;
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_spinor_in<P64>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
-    case P32: hipLaunchKernelGGL(k_spinor_in<P32>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
-    case P16: hipLaunchKernelGGL(k_spinor_in<P16>, dim3(nb), dim3(bs), 0, s, (const double2 *)ext, g.vh, dst); break;
+    case P64: hipLaunchKernelGGL(k_spinor_in<P64>, dim3(nb), dim3(bs), 0, s, e, vh, d.nrhs, rhs, v); break;
+    case P32: hipLaunchKernelGGL(k_spinor_in<P32>, dim3(nb), dim3(bs), 0, s, e, vh, d.nrhs, rhs, v); break;
+    case P16: hipLaunchKernelGGL(k_spinor_in<P16>, dim3(nb), dim3(bs), 0, s, e, vh, d.nrhs, rhs, v); break;
     default: set_error(QCU_ERR_ARG, "bad precision"); return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());
   return 0;
 }
 
-int spinor_internal_to_ext(SpinorPtr src, const Geom &g, void *ext, int prec, hipStream_t s) {
-  const int bs = 256, nb = (g.vh + bs - 1) / bs;
+int spinor_batch_to_ext(SpinorPtr src, const BatchOp &op, int seg, int rhs, void *ext, int prec, hipStream_t s) {
+  QCU_CHECK(seg >= 0 && seg < (int)op.segs.size(), QCU_ERR_ARG, "bad segment");
+  const SegDesc &d = op.segs[seg];
+  QCU_CHECK(rhs >= 0 && rhs < d.nrhs, QCU_ERR_ARG, "bad right-hand side");
+  const SpinorPtr v = seg_view(src, prec, d.voff, d.nthr);
+  const int vh = d.g.vh, bs = 256, nb = (vh + bs - 1) / bs;
+  double2 *e = (double2 *)ext;
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_spinor_out<P64>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
-    case P32: hipLaunchKernelGGL(k_spinor_out<P32>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
-    case P16: hipLaunchKernelGGL(k_spinor_out<P16>, dim3(nb), dim3(bs), 0, s, src, g.vh, (double2 *)ext); break;
+    case P64: hipLaunchKernelGGL(k_spinor_out<P64>, dim3(nb), dim3(bs), 0, s, v, vh, d.nrhs, rhs, e); break;
+    case P32: hipLaunchKernelGGL(k_spinor_out<P32>, dim3(nb), dim3(bs), 0, s, v, vh, d.nrhs, rhs, e); break;
+    case P16: hipLaunchKernelGGL(k_spinor_out<P16>, dim3(nb), dim3(bs), 0, s, v, vh, d.nrhs, rhs, e); break;
     default: set_error(QCU_ERR_ARG, "bad precision"); return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());

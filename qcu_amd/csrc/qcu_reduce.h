@@ -53,6 +53,7 @@ enum FinKindAll {
   FK_OMEGA = 6,     // omega = <t, s> / <t, t>
   FK_RHO = 7,       // rho, |r|^2, beta
   FK_RELIABLE = 8,  // after a reliable update
+  FK_STORE_TMP = 9, // st->tmp[k] = tot[k]
 };
 
 }  // namespace qcu

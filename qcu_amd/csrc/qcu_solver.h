@@ -1,19 +1,24 @@
-// qcu_solver.h -- solver work space and the preconditioner interface.
+// qcu_solver.h -- BiCGSTAB work space over a batch and the preconditioner
+// interface.
 #pragma once
 #include <map>
 #include <memory>
+#include <vector>
 
 #include "../../include/qcu.h"
 #include "qcu_lattice.h"
 
 namespace qcu {
 
-// A fixed right preconditioner K acting on odd-site spinors of one precision.
+struct Solver;
+
+// A fixed right preconditioner K acting on odd-site batch vectors of one
+// precision.
 struct Precond {
   virtual ~Precond() {}
-  // out = K in; must only enqueue work on lat.stream (it is graph-captured);
-  // st may be used to skip work once the solve has stopped.
-  virtual int apply(Lattice &lat, SpinorPtr out, SpinorPtr in, int prec, SysState *st) = 0;
+  // out = K in; must only enqueue work on S.stream (it is graph-captured);
+  // st lets kernels skip systems that have stopped.
+  virtual int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) = 0;
 };
 
 struct VecSet {
@@ -21,38 +26,44 @@ struct VecSet {
   Spinor r, rt, p, v, s, t, tmp, x, ph, sh;
   std::map<long, hipGraphExec_t> graphs;
   Precond *graph_net = nullptr;
-  int create(int prec, int px, int vh, bool precond);
+  int create(int prec, int px, int n, bool precond);
   ~VecSet();
 };
 
-struct Workspace {
-  int vh = 0;
-  Spinor be, bo, bh, xe, xo, r64, t64;
+struct Solver {
+  const BatchOp *op = nullptr;
+  hipStream_t stream = nullptr;
+  int n = 0;      // virtual sites of the batch
+  int nsys = 0;
+  Spinor be, bo, bh, xe, xo, r64, t64;  // FP64 batch vectors
   std::map<int, std::unique_ptr<VecSet>> sets;
-  int ensure(int vh);
+  DevBuf scratch;
+  double *partials = nullptr;
+  SysState *st = nullptr;    // nsys device states
+  double *dscal = nullptr;   // 4 * nsys device scalars
+  SysState *host_st = nullptr;
+  double *host_scal = nullptr;
+  int init(const BatchOp *op, hipStream_t s);
   VecSet *get_set(int prec, bool precond, int *rc);
+  ~Solver();
 };
 
 struct SolveStats {
   int iters, reliable, restarts, converged;
-  double schur_rr;
 };
 
-LaunchCfg tuned_cfg(const Lattice &lat, int prec);
+LaunchCfg tuned_cfg(const BatchOp &op, int prec);
+std::string tune_key(const BatchOp &op, int prec);
 
-int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_iter, int chunk, bool use_graph,
-                Precond *K, bool warm, SolveStats *stats);
-int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam &prm, QcuSolveInfo *info);
-
-// blas.hip, preconditioned variants
-int omega_dot(SpinorPtr t, SpinorPtr s, int prec, int vh, SysState *st, double *partials, hipStream_t strm);
-int bicg_xr_update_pc(SpinorPtr x, int px, SpinorPtr r, SpinorPtr rt, SpinorPtr ph, SpinorPtr sh, SpinorPtr s,
-                      SpinorPtr t, int prec, int vh, SysState *st, double *partials, const LaunchCfg &cfg,
-                      hipStream_t strm);
+// b[k], x[k]: full-lattice external fields of system k (device pointers)
+int full_solve(Solver &S, void *const *x_ext, const void *const *b_ext, const QcuSolveParam &prm,
+               QcuSolveInfo *info);
+// |b_k - M x_k| / |b_k| per system from external full fields
+int true_residuals(Solver &S, const void *const *x_ext, const void *const *b_ext, double *res);
 
 }  // namespace qcu
 
-// the opaque handle of the C ABI
+// the opaque handles of the C ABI
 struct QcuLattice_s {
   qcu::Lattice lat;
 };

@@ -1,4 +1,4 @@
-// runtime.hip -- errors, device buffers, resident lattice state.
+// runtime.hip -- errors, device buffers, resident lattice state, batches.
 #include <cstring>
 #include <string>
 
@@ -49,20 +49,52 @@ void DevBuf::release() {
   bytes = 0;
 }
 
-size_t spinor_bytes(int prec, int vh) {
-  switch (prec) {
-    case P64: return (size_t)vh * 12 * 16;
-    case P32: return (size_t)vh * 12 * 8;
-    default: return (size_t)vh * 12 * 4;
-  }
+size_t spinor_bytes(int prec, long long n) { return (size_t)n * words_per_site(prec) * 16; }
+
+int Spinor::create(int prec_, int n) {
+  prec = prec_;
+  vh = n;
+  QCU_TRY(data.alloc(spinor_bytes(prec, n)));
+  if (prec == P16) QCU_TRY(norm.alloc((size_t)n * sizeof(float)));
+  return 0;
 }
 
-int Spinor::create(int prec_, int vh_) {
-  prec = prec_;
-  vh = vh_;
-  QCU_TRY(data.alloc(spinor_bytes(prec, vh)));
-  if (prec == P16) QCU_TRY(norm.alloc((size_t)vh * sizeof(float)));
+// ---------------------------------------------------------------- batches
+int BatchOp::build(const std::vector<const Lattice *> &ls, const std::vector<int> &nrhs, hipStream_t s) {
+  QCU_CHECK(!ls.empty() && ls.size() == nrhs.size(), QCU_ERR_ARG, "batch: one rhs count per lattice");
+  segs.clear();
+  lats = ls;
+  nsys = 0;
+  nvsites = 0;
+  has_clover = false;
+  for (size_t k = 0; k < ls.size(); ++k) {
+    QCU_CHECK(ls[k] && ls[k]->gauge_loaded, QCU_ERR_STATE, "batch: gauge field not loaded");
+    QCU_CHECK(nrhs[k] >= 1 && nrhs[k] <= kMaxRhs, QCU_ERR_ARG, "batch: 1 to 16 right-hand sides per lattice");
+    QCU_CHECK((long long)ls[k]->g.vh * nrhs[k] < (1LL << 30), QCU_ERR_DIMS, "batch: segment too large");
+    SegDesc d = ls[k]->seg_desc(nrhs[k]);
+    d.sys0 = nsys;
+    d.voff = nvsites;
+    nsys += nrhs[k];
+    nvsites += d.nthr;
+    has_clover = has_clover || ls[k]->has_clover();
+    segs.push_back(d);
+  }
+  QCU_CHECK(!has_clover || [&] {
+    for (const Lattice *l : ls)
+      if (!l->has_clover()) return false;
+    return true;
+  }(), QCU_ERR_ARG, "batch: mixing Wilson and clover lattices is not supported");
+  QCU_TRY(dseg.alloc(segs.size() * sizeof(SegDesc)));
+  dev_segs = (SegDesc *)dseg.p;
+  QCU_HIP(hipMemcpyAsync(dev_segs, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+  QCU_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+int BatchOp::max_blocks(int bsz) const {
+  int nb = 0;
+  for (const SegDesc &d : segs) nb += (d.nthr + bsz - 1) / bsz;
+  return nb;
 }
 
 // ---------------------------------------------------------------- lattice
@@ -86,27 +118,8 @@ int Lattice::init(const int L_[4], double kappa_, double csw_, int tbc) {
   device = current_device();
   QCU_CHECK(device >= 0, QCU_ERR_STATE, "no HIP device");
   QCU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  const size_t nb = max_blocks();
-  const size_t bytes = nb * 4 * sizeof(double) + kNumStates * sizeof(SysState) + 64 * sizeof(double) + 64;
-  QCU_TRY(scratch.alloc(bytes + 1024));
-  char *base = (char *)scratch.p;
-  partials = (double *)base;
-  base += nb * 4 * sizeof(double);
-  state = (SysState *)base;
-  base += kNumStates * sizeof(SysState);
-  dscal = (double *)base;
-  base += 64 * sizeof(double);
-  bad = (int *)base;
-  QCU_HIP(hipMemsetAsync(scratch.p, 0, bytes, stream));
-  QCU_HIP(hipHostMalloc((void **)&host_state, kNumStates * sizeof(SysState), hipHostMallocDefault));
-  QCU_HIP(hipHostMalloc((void **)&host_scal, 64 * sizeof(double), hipHostMallocDefault));
-  QCU_HIP(hipStreamSynchronize(stream));
+  QCU_TRY(flag.alloc(256));
   return 0;
-}
-
-size_t Lattice::max_blocks() const {
-  // smallest block size used by any reducing kernel is 64 threads
-  return (size_t)(g.vh + 63) / 64 + 1;
 }
 
 int Lattice::load_gauge(const void *ext, int mask) {
@@ -126,6 +139,7 @@ int Lattice::load_gauge(const void *ext, int mask) {
     QCU_TRY(gauge_ext_to_internal(ext, g, P16, U16a.p, U16b.p, stream));
   }
   if (has_clover()) {
+    int *bad = (int *)flag.p;
     QCU_HIP(hipMemsetAsync(bad, 0, sizeof(int), stream));
     for (int p = 0; p < 2; ++p) {
       QCU_TRY(clov64[p].alloc(vh * 72 * 8));
@@ -148,6 +162,8 @@ int Lattice::load_gauge(const void *ext, int mask) {
   }
   QCU_HIP(hipStreamSynchronize(stream));
   gauge_loaded = true;
+  solver.reset();  // its batch table points at the previous buffers
+  QCU_TRY(self.build({this}, {1}, stream));
   return 0;
 }
 
@@ -187,51 +203,26 @@ CloverPtr Lattice::clover(int p) const {
   return c;
 }
 
-HopOp Lattice::op(int prec) const {
-  HopOp o;
-  o.U = links(prec);
-  o.C[0] = clover_inv(prec, 0);
-  o.C[1] = clover_inv(prec, 1);
-  o.g = g;
-  o.prec = prec;
-  return o;
+SegDesc Lattice::seg_desc(int nrhs) const {
+  SegDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.g = g;
+  d.nrhs = nrhs;
+  d.nthr = g.vh * nrhs;
+  d.kappa = kappa;
+  for (int p = 0; p < 3; ++p) {
+    d.U[p] = links(p);
+    d.Cinv[p][0] = clover_inv(p, 0);
+    d.Cinv[p][1] = clover_inv(p, 1);
+  }
+  d.A[0] = clover(0);
+  d.A[1] = clover(1);
+  return d;
 }
 
 Lattice::~Lattice() {
-  ws.reset();
-  if (host_state) (void)hipHostFree(host_state);
-  if (host_scal) (void)hipHostFree(host_scal);
+  solver.reset();
   if (stream) (void)hipStreamDestroy(stream);
-}
-
-// |b - M x| / |b| on the full lattice, M = A - kappa D, x and b as internal FP64 parities
-int lattice_true_residual(Lattice &lat, SpinorPtr xe, SpinorPtr xo, SpinorPtr be, SpinorPtr bo, double *res,
-                          double *bnorm, SpinorPtr scratch) {
-  Spinor own;
-  SpinorPtr t = scratch;
-  if (!t.p) {
-    QCU_TRY(own.create(P64, lat.g.vh));
-    t = own.ptr();
-  }
-  HopOp op = lat.op(P64);
-  LaunchCfg cfg = tuned_cfg(lat, P64);
-  QCU_HIP(hipMemsetAsync(lat.dscal, 0, 2 * sizeof(double), lat.stream));
-  SpinorPtr none{};
-  // even: t = D_eo x_o
-  QCU_TRY(hop(op, t, xo, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
-              false));
-  QCU_TRY(residual_parity(be, xe, t, lat.clover(0), lat.kappa, lat.g.vh, lat.partials, lat.dscal,
-                          lat.stream));
-  QCU_TRY(hop(op, t, xe, 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream,
-              false));
-  QCU_TRY(residual_parity(bo, xo, t, lat.clover(1), lat.kappa, lat.g.vh, lat.partials, lat.dscal,
-                          lat.stream));
-  QCU_HIP(hipMemcpyAsync(lat.host_scal, lat.dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, lat.stream));
-  QCU_HIP(hipStreamSynchronize(lat.stream));
-  const double num = lat.host_scal[0], den = lat.host_scal[1];
-  *res = den > 0 ? sqrt(num / den) : sqrt(num);
-  if (bnorm) *bnorm = sqrt(den);
-  return 0;
 }
 
 }  // namespace qcu

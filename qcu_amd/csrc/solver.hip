@@ -1,14 +1,16 @@
-// solver.hip -- even-odd BiCGSTAB on the odd-site Schur system, FP64 or
-// mixed precision with FP64 reliable updates.
+// solver.hip -- even-odd BiCGSTAB on the odd-site Schur system for a batch of
+// systems, FP64 or mixed precision with FP64 reliable updates.
 //
 // Schur form (symmetric, reducing to include/lattice_bistabcg.h:99-132 and
 // src/bistabcg.cu:73-96 for Wilson, A = 1):
 //   Mhat x_o = x_o - kappa^2 Ao^-1 D_oe Ae^-1 D_eo x_o
 //   bhat     = Ao^-1 (b_o + kappa D_oe Ae^-1 b_e)
 //   x_e      = Ae^-1 (b_e + kappa D_eo x_o)
-// The recurrence is the reference's (include/lattice_bistabcg.h:227-307); its
-// scalars stay on the device and the host looks at them once per chunk of
-// iterations.  A chunk is captured once as a hipGraph and replayed.
+// The recurrence is the reference's (include/lattice_bistabcg.h:227-307).  Its
+// scalars stay on the device, one state per system; the host looks at them
+// once per chunk of iterations, and a chunk is captured once as a hipGraph and
+// replayed.  Systems that stop are skipped by every kernel until the batch
+// is done.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -26,7 +28,8 @@ struct PhaseTrace {
   bool on;
   hipStream_t s;
   std::chrono::steady_clock::time_point t;
-  explicit PhaseTrace(hipStream_t st) : on(getenv("QCU_TRACE") != nullptr), s(st), t(std::chrono::steady_clock::now()) {}
+  explicit PhaseTrace(hipStream_t st)
+      : on(getenv("QCU_TRACE") != nullptr), s(st), t(std::chrono::steady_clock::now()) {}
   void mark(const char *what) {
     if (!on) return;
     (void)hipStreamSynchronize(s);
@@ -36,333 +39,428 @@ struct PhaseTrace {
   }
 };
 
-int VecSet::create(int prec_, int px_, int vh, bool precond) {
+// ---------------------------------------------------------------- work space
+int VecSet::create(int prec_, int px_, int n, bool precond) {
   prec = prec_;
   px = px_;
-  QCU_TRY(r.create(prec, vh));
-  QCU_TRY(rt.create(prec, vh));
-  QCU_TRY(p.create(prec, vh));
-  QCU_TRY(v.create(prec, vh));
-  QCU_TRY(s.create(prec, vh));
-  QCU_TRY(t.create(prec, vh));
-  QCU_TRY(tmp.create(prec, vh));
-  QCU_TRY(x.create(px, vh));
+  QCU_TRY(r.create(prec, n));
+  QCU_TRY(rt.create(prec, n));
+  QCU_TRY(p.create(prec, n));
+  QCU_TRY(v.create(prec, n));
+  QCU_TRY(s.create(prec, n));
+  QCU_TRY(t.create(prec, n));
+  QCU_TRY(tmp.create(prec, n));
+  QCU_TRY(x.create(px, n));
   if (precond) {
-    QCU_TRY(ph.create(prec, vh));
-    QCU_TRY(sh.create(prec, vh));
+    QCU_TRY(ph.create(prec, n));
+    QCU_TRY(sh.create(prec, n));
   }
   return 0;
 }
 
 VecSet::~VecSet() {
-  for (auto &kv : graphs) {
-    (void)hipGraphExecDestroy(kv.second);
-  }
+  for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
 }
 
-int Workspace::ensure(int vh_) {
-  if (vh == vh_) return 0;
-  vh = vh_;
-  QCU_TRY(be.create(P64, vh));
-  QCU_TRY(bo.create(P64, vh));
-  QCU_TRY(bh.create(P64, vh));
-  QCU_TRY(xe.create(P64, vh));
-  QCU_TRY(xo.create(P64, vh));
-  QCU_TRY(r64.create(P64, vh));
-  QCU_TRY(t64.create(P64, vh));
-  sets.clear();
+int Solver::init(const BatchOp *op_, hipStream_t s) {
+  op = op_;
+  stream = s;
+  QCU_CHECK(op->nvsites < (1LL << 31), QCU_ERR_DIMS, "batch too large");
+  n = (int)op->nvsites;
+  nsys = op->nsys;
+  QCU_TRY(be.create(P64, n));
+  QCU_TRY(bo.create(P64, n));
+  QCU_TRY(bh.create(P64, n));
+  QCU_TRY(xe.create(P64, n));
+  QCU_TRY(xo.create(P64, n));
+  QCU_TRY(r64.create(P64, n));
+  QCU_TRY(t64.create(P64, n));
+  // partials of the smallest block size (64) used by any reducing kernel
+  const size_t npart = (size_t)(op->max_blocks(64) + 1) * kMaxRhs * 4;
+  const size_t bytes = npart * sizeof(double) + nsys * sizeof(SysState) + 4 * nsys * sizeof(double) + 256;
+  QCU_TRY(scratch.alloc(bytes));
+  char *base = (char *)scratch.p;
+  partials = (double *)base;
+  base += npart * sizeof(double);
+  st = (SysState *)base;
+  base += nsys * sizeof(SysState);
+  dscal = (double *)base;
+  QCU_HIP(hipMemsetAsync(scratch.p, 0, bytes, stream));
+  QCU_HIP(hipHostMalloc((void **)&host_st, nsys * sizeof(SysState), hipHostMallocDefault));
+  QCU_HIP(hipHostMalloc((void **)&host_scal, 4 * nsys * sizeof(double), hipHostMallocDefault));
+  QCU_HIP(hipStreamSynchronize(stream));
   return 0;
 }
 
-VecSet *Workspace::get_set(int prec, bool precond, int *rc) {
-  int key = prec * 2 + (precond ? 1 : 0);
+Solver::~Solver() {
+  sets.clear();
+  if (host_st) (void)hipHostFree(host_st);
+  if (host_scal) (void)hipHostFree(host_scal);
+}
+
+VecSet *Solver::get_set(int prec, bool precond, int *rc) {
+  const int key = prec * 2 + (precond ? 1 : 0);
   auto it = sets.find(key);
   if (it != sets.end()) return it->second.get();
   std::unique_ptr<VecSet> vs(new VecSet());
-  int px = (prec == P64) ? P64 : P32;
-  *rc = vs->create(prec, px, vh, precond);
+  const int px = (prec == P64) ? P64 : P32;
+  *rc = vs->create(prec, px, n, precond);
   if (*rc) return nullptr;
   VecSet *raw = vs.get();
   sets[key] = std::move(vs);
   return raw;
 }
 
-// Mhat x (FP64) into out; t64 as scratch
-static int apply_mhat64(Lattice &lat, SpinorPtr out, SpinorPtr x, SpinorPtr tmp) {
-  HopOp op = lat.op(P64);
-  LaunchCfg cfg = tuned_cfg(lat, P64);
+// ---------------------------------------------------------------- operators
+// Mhat x (FP64) into out; tmp is scratch
+static int apply_mhat64(Solver &S, SpinorPtr out, SpinorPtr x, SpinorPtr tmp) {
+  const BatchOp &op = *S.op;
+  const LaunchCfg cfg = tuned_cfg(op, P64);
   SpinorPtr none{};
-  QCU_TRY(hop(op, tmp, x, 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg, lat.stream));
-  QCU_TRY(hop(op, out, tmp, 1, EPI_XPAY, x, lat.kappa * lat.kappa, RED_NONE, none, nullptr, nullptr,
-              cfg, lat.stream));
+  QCU_TRY(hop(op, P64, tmp, x, 0, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, S.stream));
+  QCU_TRY(hop(op, P64, out, tmp, 1, EPI_XPAY, x, RED_NONE, none, nullptr, nullptr, cfg, S.stream));
   return 0;
 }
 
-// one BiCGSTAB iteration on the device, no host interaction
-static int enqueue_iteration(Lattice &lat, VecSet &S, SysState *st, Precond *K) {
-  const int prec = S.prec, vh = lat.g.vh;
-  HopOp op = lat.op(prec);
-  LaunchCfg cfg = tuned_cfg(lat, prec);
-  LaunchCfg bcfg;
-  bcfg.block = 256;
+// one BiCGSTAB iteration of every running system, no host interaction
+static int enqueue_iteration(Solver &S, VecSet &V, Precond *K) {
+  const BatchOp &op = *S.op;
+  const int prec = V.prec;
+  const LaunchCfg cfg = tuned_cfg(op, prec);
+  SysState *st = S.st;
   SpinorPtr none{};
-  const double k2 = lat.kappa * lat.kappa;
-  QCU_TRY(bicg_p_update(S.p.ptr(), S.r.ptr(), S.v.ptr(), prec, vh, st, bcfg, lat.stream));
-  SpinorPtr pin = S.p.ptr();
+  QCU_TRY(bicg_p_update(op, V.p.ptr(), V.r.ptr(), V.v.ptr(), prec, st, S.stream));
+  SpinorPtr pin = V.p.ptr();
   if (K) {
-    QCU_TRY(K->apply(lat, S.ph.ptr(), S.p.ptr(), prec, st));
-    pin = S.ph.ptr();
+    QCU_TRY(K->apply(S, V.ph.ptr(), V.p.ptr(), prec, st));
+    pin = V.ph.ptr();
   }
-  QCU_TRY(hop(op, S.tmp.ptr(), pin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, cfg, lat.stream));
-  QCU_TRY(hop(op, S.v.ptr(), S.tmp.ptr(), 1, EPI_XPAY, pin, k2, RED_DOT_Y_OUT, S.rt.ptr(), st, lat.partials,
-              cfg, lat.stream));
-  QCU_TRY(bicg_s_update(S.s.ptr(), S.r.ptr(), S.v.ptr(), prec, vh, st, bcfg, lat.stream));
-  SpinorPtr sin = S.s.ptr();
+  QCU_TRY(hop(op, prec, V.tmp.ptr(), pin, 0, EPI_PLAIN, none, RED_NONE, none, st, nullptr, cfg, S.stream));
+  QCU_TRY(hop(op, prec, V.v.ptr(), V.tmp.ptr(), 1, EPI_XPAY, pin, RED_DOT_Y_OUT, V.rt.ptr(), st, S.partials, cfg,
+              S.stream));
+  QCU_TRY(bicg_s_update(op, V.s.ptr(), V.r.ptr(), V.v.ptr(), prec, st, S.stream));
+  SpinorPtr sin = V.s.ptr();
   if (K) {
-    QCU_TRY(K->apply(lat, S.sh.ptr(), S.s.ptr(), prec, st));
-    sin = S.sh.ptr();
+    QCU_TRY(K->apply(S, V.sh.ptr(), V.s.ptr(), prec, st));
+    sin = V.sh.ptr();
   }
-  QCU_TRY(hop(op, S.tmp.ptr(), sin, 0, EPI_PLAIN, none, 0, RED_NONE, none, st, nullptr, cfg, lat.stream));
+  QCU_TRY(hop(op, prec, V.tmp.ptr(), sin, 0, EPI_PLAIN, none, RED_NONE, none, st, nullptr, cfg, S.stream));
   if (K) {
-    // t = Mhat s_hat; <t, s> needs s, not s_hat: reduce in the update kernel below
-    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_NONE, none, st, nullptr, cfg,
-                lat.stream));
-    QCU_TRY(omega_dot(S.t.ptr(), S.s.ptr(), prec, vh, st, lat.partials, lat.stream));
-    QCU_TRY(bicg_xr_update_pc(S.x.ptr(), S.px, S.r.ptr(), S.rt.ptr(), S.ph.ptr(), S.sh.ptr(), S.s.ptr(),
-                              S.t.ptr(), prec, vh, st, lat.partials, bcfg, lat.stream));
+    // t = Mhat s_hat; omega = <t, s> / <t, t> needs s, not s_hat
+    QCU_TRY(hop(op, prec, V.t.ptr(), V.tmp.ptr(), 1, EPI_XPAY, sin, RED_NONE, none, st, nullptr, cfg, S.stream));
+    QCU_TRY(omega_dot(op, V.t.ptr(), V.s.ptr(), prec, st, S.partials, S.stream));
+    QCU_TRY(bicg_xr_update_pc(op, V.x.ptr(), V.px, V.r.ptr(), V.rt.ptr(), V.ph.ptr(), V.sh.ptr(), V.s.ptr(),
+                              V.t.ptr(), prec, st, S.partials, S.stream));
   } else {
-    QCU_TRY(hop(op, S.t.ptr(), S.tmp.ptr(), 1, EPI_XPAY, sin, k2, RED_OUT_X, S.s.ptr(), st, lat.partials,
-                cfg, lat.stream));
-    QCU_TRY(bicg_xr_update(S.x.ptr(), S.px, S.r.ptr(), S.rt.ptr(), S.p.ptr(), S.s.ptr(), S.t.ptr(), prec, vh,
-                           st, lat.partials, bcfg, lat.stream));
+    QCU_TRY(hop(op, prec, V.t.ptr(), V.tmp.ptr(), 1, EPI_XPAY, sin, RED_OUT_X, none, st, S.partials, cfg,
+                S.stream));
+    QCU_TRY(bicg_xr_update(op, V.x.ptr(), V.px, V.r.ptr(), V.rt.ptr(), V.p.ptr(), V.s.ptr(), V.t.ptr(), prec, st,
+                           S.partials, S.stream));
   }
   return 0;
 }
 
-static int run_chunk(Lattice &lat, VecSet &S, SysState *st, Precond *K, int chunk, bool use_graph) {
+static int run_chunk(Solver &S, VecSet &V, Precond *K, int chunk, bool use_graph) {
   if (!use_graph) {
-    for (int k = 0; k < chunk; ++k) QCU_TRY(enqueue_iteration(lat, S, st, K));
+    for (int k = 0; k < chunk; ++k) QCU_TRY(enqueue_iteration(S, V, K));
     return 0;
   }
   const long key = ((long)chunk << 8) | (K ? 1 : 0);
-  auto it = S.graphs.find(key);
-  if (it == S.graphs.end() || S.graph_net != K) {
-    if (it != S.graphs.end()) {
-      (void)hipGraphExecDestroy(it->second);
-      S.graphs.erase(it);
-    }
+  auto it = V.graphs.find(key);
+  if (it != V.graphs.end() && V.graph_net != K) {
+    (void)hipGraphExecDestroy(it->second);
+    V.graphs.erase(it);
+    it = V.graphs.end();
+  }
+  if (it == V.graphs.end()) {
     hipGraph_t graph;
-    QCU_HIP(hipStreamBeginCapture(lat.stream, hipStreamCaptureModeThreadLocal));
+    QCU_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int k = 0; k < chunk && !rc; ++k) rc = enqueue_iteration(lat, S, st, K);
-    hipError_t e = hipStreamEndCapture(lat.stream, &graph);
+    for (int k = 0; k < chunk && !rc; ++k) rc = enqueue_iteration(S, V, K);
+    hipError_t e = hipStreamEndCapture(S.stream, &graph);
     if (rc) return rc;
     QCU_HIP(e);
     hipGraphExec_t exec;
     QCU_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
-    S.graphs[key] = exec;
-    S.graph_net = K;
-    it = S.graphs.find(key);
+    V.graphs[key] = exec;
+    V.graph_net = K;
+    it = V.graphs.find(key);
   }
-  QCU_HIP(hipGraphLaunch(it->second, lat.stream));
+  QCU_HIP(hipGraphLaunch(it->second, S.stream));
   return 0;
 }
 
-static int read_state(Lattice &lat, SysState *st, SysState *host) {
-  QCU_HIP(hipMemcpyAsync(host, st, sizeof(SysState), hipMemcpyDeviceToHost, lat.stream));
-  QCU_HIP(hipStreamSynchronize(lat.stream));
+static int read_states(Solver &S) {
+  QCU_HIP(hipMemcpyAsync(S.host_st, S.st, S.nsys * sizeof(SysState), hipMemcpyDeviceToHost, S.stream));
+  QCU_HIP(hipStreamSynchronize(S.stream));
   return 0;
 }
 
-static int write_state(Lattice &lat, SysState *st, const SysState *host) {
-  QCU_HIP(hipMemcpyAsync(st, host, sizeof(SysState), hipMemcpyHostToDevice, lat.stream));
+static int write_states(Solver &S) {
+  QCU_HIP(hipMemcpyAsync(S.st, S.host_st, S.nsys * sizeof(SysState), hipMemcpyHostToDevice, S.stream));
   return 0;
 }
 
-// Solve Mhat x_o = bhat (bhat in ws.bh); the solution accumulates in ws.xo
-// (FP64): the iterated correction S.x is folded into it at every reliable
-// update.  warm: start from the current ws.xo instead of zero.
-// target2: stop when |bhat - Mhat x_o|^2 < target2 (recomputed in FP64).
-int schur_solve(Lattice &lat, int prec, double target2, double delta, int max_iter, int chunk, bool use_graph,
-                Precond *K, bool warm, SolveStats *stats) {
-  Workspace &ws = *lat.ws;
-  const int vh = lat.g.vh;
+// Solve Mhat x_o = bhat (bhat in S.bh) for the systems with active[k]; the
+// solution accumulates in S.xo (FP64): the iterated correction V.x is folded
+// into it at every reliable update.  warm: start from the current S.xo.
+// A system stops when |bhat - Mhat x_o|^2 < target2[k] (recomputed in FP64).
+static int schur_solve(Solver &S, int prec, const std::vector<double> &target2, double delta, int max_iter,
+                       int chunk, bool use_graph, Precond *K, bool warm, const std::vector<int> &active,
+                       std::vector<SolveStats> &stats) {
+  const BatchOp &op = *S.op;
+  const int nsys = S.nsys;
   int rc = 0;
-  VecSet *S = ws.get_set(prec, K != nullptr, &rc);
-  if (!S) return rc;
-  SysState *st = lat.state;
-  SysState *hs = lat.host_state;
-  std::memset(hs, 0, sizeof(SysState));
-  hs->rr_stop = target2;
-  hs->delta2 = (prec == P64) ? 0.0 : delta * delta;
-  QCU_TRY(write_state(lat, st, hs));
-  LaunchCfg bcfg;
-  SpinorPtr src = ws.bh.ptr();
-  if (warm) {
-    QCU_TRY(apply_mhat64(lat, ws.r64.ptr(), ws.xo.ptr(), ws.t64.ptr()));
-    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, FIN_NONE,
-                     lat.stream));
-    src = ws.r64.ptr();
-  } else {
-    QCU_TRY(zero_spinor(ws.xo.ptr(), P64, vh, lat.stream));
+  VecSet *V = S.get_set(prec, K != nullptr, &rc);
+  if (!V) return rc;
+  SysState *hs = S.host_st;
+  std::memset(hs, 0, nsys * sizeof(SysState));
+  for (int k = 0; k < nsys; ++k) {
+    hs[k].rr_stop = target2[k];
+    hs[k].delta2 = (prec == P64) ? 0.0 : delta * delta;
+    hs[k].done = active[k] ? 0 : 4;
   }
-  // S.x = 0, r = src, r~ = r, p = v = 0
-  QCU_TRY(bicg_init(S->r.ptr(), S->rt.ptr(), S->p.ptr(), S->v.ptr(), S->x.ptr(), prec, src, P64, vh, st,
-                    lat.partials, bcfg, lat.stream));
-  QCU_TRY(read_state(lat, st, hs));
-  PhaseTrace trace(lat.stream);
+  QCU_TRY(write_states(S));
+  SpinorPtr src = S.bh.ptr();
+  if (warm) {
+    QCU_TRY(apply_mhat64(S, S.r64.ptr(), S.xo.ptr(), S.t64.ptr()));
+    QCU_TRY(xmy_norm(op, S.bh.ptr(), S.r64.ptr(), S.st, S.partials, 0, S.stream));
+    src = S.r64.ptr();
+  } else {
+    QCU_TRY(zero_spinor(op, S.xo.ptr(), P64, S.stream));
+  }
+  // V.x = 0, r = src, r~ = r, p = v = 0
+  QCU_TRY(bicg_init(op, V->r.ptr(), V->rt.ptr(), V->p.ptr(), V->v.ptr(), V->x.ptr(), prec, src, S.st, S.partials,
+                    S.stream));
+  QCU_TRY(read_states(S));
+  PhaseTrace trace(S.stream);
   trace.mark("  init");
-  int iters = 0, reliable = 0, stuck = 0, restarts = 0;
-  bool converged = false;
-  double last_true = hs->rr;
+  std::vector<int> stuck(nsys, 0);
+  for (int k = 0; k < nsys; ++k) stats[k] = SolveStats{0, 0, 0, 0};
+  int iters = 0;
   for (;;) {
-    if (hs->done == 0) {
+    bool running = false, pending = false;
+    for (int k = 0; k < nsys; ++k) {
+      running = running || hs[k].done == 0;
+      pending = pending || (hs[k].done >= 1 && hs[k].done <= 3);
+    }
+    if (!running && !pending) break;
+    if (!pending) {
       const int n = std::min(chunk, max_iter - iters);
       if (n <= 0) break;
-      QCU_TRY(run_chunk(lat, *S, st, K, n, use_graph && n == chunk));
-      QCU_TRY(read_state(lat, st, hs));
-      iters = hs->iter;
+      QCU_TRY(run_chunk(S, *V, K, n, use_graph && n == chunk));
+      QCU_TRY(read_states(S));
+      for (int k = 0; k < nsys; ++k) iters = std::max(iters, hs[k].iter);
       trace.mark("  chunk");
       continue;
     }
-    const int why = hs->done;
-    if (why == 2 && ++restarts > 20) break;
-    // reliable update: fold the correction into the FP64 solution and
-    // recompute the residual in FP64 (also the convergence check)
-    QCU_TRY(axpby(1.0, S->x.ptr(), S->px, 1.0, ws.xo.ptr(), P64, vh, lat.stream));
-    QCU_TRY(zero_spinor(S->x.ptr(), S->px, vh, lat.stream));
-    QCU_TRY(apply_mhat64(lat, ws.r64.ptr(), ws.xo.ptr(), ws.t64.ptr()));
-    QCU_TRY(xmy_norm(ws.bh.ptr(), P64, ws.r64.ptr(), P64, vh, st, lat.partials, FIN_STORE,
-                     lat.stream));
-    QCU_TRY(read_state(lat, st, hs));
-    const double rr_true = hs->tmp[0];
-    ++reliable;
-    if (rr_true < target2) {
-      converged = true;
-      last_true = rr_true;
-      break;
-    }
-    if (iters >= max_iter) {
-      last_true = rr_true;
-      break;
-    }
-    if (why == 1) {
-      // the iterated residual claimed convergence but the true one did not:
-      // ask the recurrence for more (guards against a precision floor)
-      if (++stuck > 8) {
-        last_true = rr_true;
-        break;
+    // reliable update of every system still in the batch: fold the
+    // correction into the FP64 solution and recompute the residual in FP64
+    // (this is also the convergence check)
+    std::vector<int> why(nsys);
+    for (int k = 0; k < nsys; ++k) why[k] = hs[k].done;
+    QCU_TRY(axpby(op, 1.0, V->x.ptr(), V->px, 1.0, S.xo.ptr(), P64, S.stream));
+    QCU_TRY(zero_spinor(op, V->x.ptr(), V->px, S.stream));
+    QCU_TRY(apply_mhat64(S, S.r64.ptr(), S.xo.ptr(), S.t64.ptr()));
+    QCU_TRY(xmy_norm(op, S.bh.ptr(), S.r64.ptr(), S.st, S.partials, 1, S.stream));
+    QCU_TRY(read_states(S));
+    bool restart = false;
+    for (int k = 0; k < nsys; ++k) {
+      if (why[k] == 4) continue;
+      stats[k].reliable += 1;
+      stats[k].iters = hs[k].iter;
+      const double rr_true = hs[k].tmp[0];
+      int d = 0;
+      if (rr_true < target2[k]) {
+        d = 4;
+        stats[k].converged = 1;
+      } else if (hs[k].iter >= max_iter || !std::isfinite(rr_true)) {
+        d = 4;
+      } else if (why[k] == 1 && ++stuck[k] > 8) {
+        d = 4;  // the iterated residual keeps claiming convergence: a precision floor
+      } else if (why[k] == 2 && ++stats[k].restarts > 20) {
+        d = 4;
       }
-      hs->rr_stop = std::min(hs->rr_stop, target2) * 0.01;
+      if (d == 0 && why[k] == 1) hs[k].rr_stop = std::min(hs[k].rr_stop, target2[k]) * 0.01;
+      if (d == 0 && why[k] == 2) restart = true;
+      hs[k].done = d;
     }
-    last_true = rr_true;
-    QCU_TRY(spinor_convert(S->r.ptr(), prec, ws.r64.ptr(), P64, vh, lat.stream));
-    if (why == 2) {
-      // breakdown: restart the Krylov space from the current residual
-      QCU_TRY(spinor_convert(S->rt.ptr(), prec, ws.r64.ptr(), P64, vh, lat.stream));
-      QCU_TRY(zero_spinor(S->p.ptr(), prec, vh, lat.stream));
-      QCU_TRY(zero_spinor(S->v.ptr(), prec, vh, lat.stream));
-      hs->rho_prev[0] = 1;
-      hs->rho_prev[1] = 0;
-      hs->alpha[0] = hs->omega[0] = 1;
-      hs->alpha[1] = hs->omega[1] = 0;
+    QCU_TRY(spinor_convert(op, V->r.ptr(), prec, S.r64.ptr(), P64, S.stream));
+    if (restart) {
+      // a breakdown restarts the Krylov spaces of the batch from the current residuals
+      QCU_TRY(spinor_convert(op, V->rt.ptr(), prec, S.r64.ptr(), P64, S.stream));
+      QCU_TRY(zero_spinor(op, V->p.ptr(), prec, S.stream));
+      QCU_TRY(zero_spinor(op, V->v.ptr(), prec, S.stream));
+      for (int k = 0; k < nsys; ++k) {
+        hs[k].rho_prev[0] = 1;
+        hs[k].rho_prev[1] = 0;
+        hs[k].alpha[0] = hs[k].omega[0] = 1;
+        hs[k].alpha[1] = hs[k].omega[1] = 0;
+      }
     }
-    hs->done = 0;
-    QCU_TRY(write_state(lat, st, hs));
-    QCU_TRY(reliable_dot(S->rt.ptr(), S->r.ptr(), prec, vh, st, lat.partials, lat.stream));
-    QCU_TRY(read_state(lat, st, hs));
+    QCU_TRY(write_states(S));
+    QCU_TRY(reliable_dot(op, V->rt.ptr(), V->r.ptr(), prec, S.st, S.partials, S.stream));
+    QCU_TRY(read_states(S));
     trace.mark("  reliable update");
   }
-  QCU_TRY(axpby(1.0, S->x.ptr(), S->px, 1.0, ws.xo.ptr(), P64, vh, lat.stream));
-  QCU_TRY(zero_spinor(S->x.ptr(), S->px, vh, lat.stream));
-  stats->iters = iters;
-  stats->reliable = reliable;
-  stats->restarts = restarts;
-  stats->converged = converged;
-  stats->schur_rr = last_true;
+  QCU_TRY(axpby(op, 1.0, V->x.ptr(), V->px, 1.0, S.xo.ptr(), P64, S.stream));
+  QCU_TRY(zero_spinor(op, V->x.ptr(), V->px, S.stream));
+  for (int k = 0; k < nsys; ++k) stats[k].iters = std::max(stats[k].iters, hs[k].iter);
   return 0;
 }
 
-// Full solve: b (external full, device) -> x (external full, device)
-int full_solve(Lattice &lat, void *x_ext, const void *b_ext, const QcuSolveParam &prm, QcuSolveInfo *info) {
-  QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
-  if (!lat.ws) lat.ws.reset(new Workspace());
-  Workspace &ws = *lat.ws;
-  QCU_TRY(ws.ensure(lat.g.vh));
-  const int vh = lat.g.vh;
-  auto t0 = std::chrono::steady_clock::now();
-  PhaseTrace trace(lat.stream);
-  const char *b = (const char *)b_ext;
-  char *x = (char *)x_ext;
-  const size_t half = (size_t)vh * 12 * 16;
-  QCU_TRY(spinor_ext_to_internal(b, lat.g, ws.be.ptr(), P64, lat.stream));
-  QCU_TRY(spinor_ext_to_internal(b + half, lat.g, ws.bo.ptr(), P64, lat.stream));
-  // bhat = Ao^-1 (b_o + kappa D_oe Ae^-1 b_e)
-  HopOp op = lat.op(P64);
-  LaunchCfg cfg = tuned_cfg(lat, P64);
+// per-system |b - M x|^2 and |b|^2 of the internal FP64 fields into S.host_scal[2k], [2k+1]
+static int residuals_internal(Solver &S, SpinorPtr xe, SpinorPtr xo, SpinorPtr be, SpinorPtr bo, SpinorPtr t) {
+  const BatchOp &op = *S.op;
+  const LaunchCfg cfg = tuned_cfg(op, P64);
   SpinorPtr none{};
-  QCU_TRY(cl_axpy(ws.t64.ptr(), ws.be.ptr(), 0.0, none, lat.clover_inv(P64, 0), vh, lat.stream));
-  QCU_TRY(hop(op, ws.r64.ptr(), ws.t64.ptr(), 1, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg,
-              lat.stream, false));
-  QCU_TRY(cl_axpy(ws.bh.ptr(), ws.bo.ptr(), lat.kappa, ws.r64.ptr(), lat.clover_inv(P64, 1), vh, lat.stream));
-  // |b|^2 for the stopping rule
-  QCU_TRY(norm2_dev(ws.be.ptr(), P64, vh, lat.dscal, lat.partials, lat.stream));
-  QCU_TRY(norm2_dev(ws.bo.ptr(), P64, vh, lat.dscal + 1, lat.partials, lat.stream));
-  QCU_TRY(norm2_dev(ws.bh.ptr(), P64, vh, lat.dscal + 2, lat.partials, lat.stream));
-  QCU_HIP(hipMemcpyAsync(lat.host_scal, lat.dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, lat.stream));
-  QCU_HIP(hipStreamSynchronize(lat.stream));
-  const double b2 = lat.host_scal[0] + lat.host_scal[1];
-  const double bh2 = lat.host_scal[2];
+  QCU_HIP(hipMemsetAsync(S.dscal, 0, 2 * S.nsys * sizeof(double), S.stream));
+  QCU_TRY(hop(op, P64, t, xo, 0, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, S.stream, false));
+  QCU_TRY(residual_parity(op, be, xe, t, 0, S.partials, S.dscal, S.stream));
+  QCU_TRY(hop(op, P64, t, xe, 1, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, S.stream, false));
+  QCU_TRY(residual_parity(op, bo, xo, t, 1, S.partials, S.dscal, S.stream));
+  QCU_HIP(hipMemcpyAsync(S.host_scal, S.dscal, 2 * S.nsys * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+  QCU_HIP(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+static void system_of(const BatchOp &op, int k, int *seg, int *rhs) {
+  for (size_t s = 0; s < op.segs.size(); ++s) {
+    if (k >= op.segs[s].sys0 && k < op.segs[s].sys0 + op.segs[s].nrhs) {
+      *seg = (int)s;
+      *rhs = k - op.segs[s].sys0;
+      return;
+    }
+  }
+  *seg = -1;
+  *rhs = 0;
+}
+
+static int load_systems(Solver &S, const void *const *ext, SpinorPtr even, SpinorPtr odd) {
+  const BatchOp &op = *S.op;
+  for (int k = 0; k < S.nsys; ++k) {
+    int seg, rhs;
+    system_of(op, k, &seg, &rhs);
+    const size_t half = (size_t)op.segs[seg].g.vh * 12 * 16;
+    QCU_TRY(spinor_ext_to_batch(ext[k], op, seg, rhs, even, P64, S.stream));
+    QCU_TRY(spinor_ext_to_batch((const char *)ext[k] + half, op, seg, rhs, odd, P64, S.stream));
+  }
+  return 0;
+}
+
+static int store_systems(Solver &S, void *const *ext, SpinorPtr even, SpinorPtr odd) {
+  const BatchOp &op = *S.op;
+  for (int k = 0; k < S.nsys; ++k) {
+    int seg, rhs;
+    system_of(op, k, &seg, &rhs);
+    const size_t half = (size_t)op.segs[seg].g.vh * 12 * 16;
+    QCU_TRY(spinor_batch_to_ext(even, op, seg, rhs, ext[k], P64, S.stream));
+    QCU_TRY(spinor_batch_to_ext(odd, op, seg, rhs, (char *)ext[k] + half, P64, S.stream));
+  }
+  return 0;
+}
+
+int true_residuals(Solver &S, const void *const *x_ext, const void *const *b_ext, double *res) {
+  QCU_TRY(load_systems(S, x_ext, S.xe.ptr(), S.xo.ptr()));
+  QCU_TRY(load_systems(S, b_ext, S.be.ptr(), S.bo.ptr()));
+  QCU_TRY(residuals_internal(S, S.xe.ptr(), S.xo.ptr(), S.be.ptr(), S.bo.ptr(), S.r64.ptr()));
+  for (int k = 0; k < S.nsys; ++k) {
+    const double num = S.host_scal[2 * k], den = S.host_scal[2 * k + 1];
+    res[k] = den > 0 ? sqrt(num / den) : sqrt(num);
+  }
+  return 0;
+}
+
+int full_solve(Solver &S, void *const *x_ext, const void *const *b_ext, const QcuSolveParam &prm, QcuSolveInfo *info) {
+  const BatchOp &op = *S.op;
+  const int nsys = S.nsys;
+  auto t0 = std::chrono::steady_clock::now();
+  PhaseTrace trace(S.stream);
+  QCU_TRY(load_systems(S, b_ext, S.be.ptr(), S.bo.ptr()));
+  // bhat = Ao^-1 (b_o + kappa D_oe Ae^-1 b_e)
+  const LaunchCfg cfg = tuned_cfg(op, P64);
+  SpinorPtr none{};
+  QCU_TRY(cl_axpy(op, S.t64.ptr(), S.be.ptr(), 0.0, false, none, 1, 0, S.stream));
+  QCU_TRY(hop(op, P64, S.r64.ptr(), S.t64.ptr(), 1, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, S.stream,
+              false));
+  QCU_TRY(cl_axpy(op, S.bh.ptr(), S.bo.ptr(), 0.0, true, S.r64.ptr(), 1, 1, S.stream));
+  // |b|^2 per system for the stopping rule
+  QCU_TRY(norm2_dev(op, S.be.ptr(), P64, S.dscal, S.partials, S.stream));
+  QCU_TRY(norm2_dev(op, S.bo.ptr(), P64, S.dscal + nsys, S.partials, S.stream));
+  QCU_HIP(hipMemcpyAsync(S.host_scal, S.dscal, 2 * nsys * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+  QCU_HIP(hipStreamSynchronize(S.stream));
+  std::vector<double> target2(nsys), res(nsys, INFINITY);
+  for (int k = 0; k < nsys; ++k) {
+    const double b2 = S.host_scal[k] + S.host_scal[nsys + k];
+    // legacy: Re<r,r> < tol on the Schur residual, as the reference
+    target2[k] = prm.legacy_stop ? prm.tol : 0.25 * prm.tol * prm.tol * b2;
+  }
   trace.mark("b_hat + norms");
-  double target2;
-  if (prm.legacy_stop)
-    target2 = prm.tol;  // Re<r,r> < tol on the Schur residual, as the reference
-  else
-    target2 = 0.25 * prm.tol * prm.tol * b2;
   const int chunk = prm.chunk > 0 ? prm.chunk : 8;
   const int max_iter = prm.max_iter > 0 ? prm.max_iter : 10000;
   const double delta = prm.reliable_delta > 0 ? prm.reliable_delta : 0.1;
   Precond *K = (Precond *)prm.precond;
-  SolveStats st{};
-  int total_iters = 0, total_rel = 0, fallback = 0, restarts = 0;
-  double res = INFINITY;
-  for (int attempt = 0; attempt < 6; ++attempt) {
-    QCU_TRY(schur_solve(lat, prm.precision, target2, delta, max_iter - total_iters, chunk, prm.use_graph != 0, K,
-                        attempt > 0, &st));
+  std::vector<int> active(nsys, 1), fallback(nsys, 0), total_iters(nsys, 0), total_rel(nsys, 0),
+      restarts(nsys, 0), converged(nsys, 0);
+  std::vector<SolveStats> st(nsys);
+  bool warm = false;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    int budget = max_iter;
+    for (int k = 0; k < nsys; ++k)
+      if (active[k]) budget = std::min(budget, max_iter - total_iters[k]);
+    QCU_TRY(schur_solve(S, prm.precision, target2, delta, std::max(budget, 0), chunk, prm.use_graph != 0, K, warm,
+                        active, st));
     trace.mark("schur solve");
-    total_iters += st.iters;
-    total_rel += st.reliable;
-    restarts += st.restarts;
-    if (!st.converged && K) {
-      // stalled preconditioned solve: fall back to the plain method from the current x_o
+    bool stalled = false;
+    for (int k = 0; k < nsys; ++k) {
+      if (!active[k]) continue;
+      total_iters[k] += st[k].iters;
+      total_rel[k] += st[k].reliable;
+      restarts[k] += st[k].restarts;
+      stalled = stalled || !st[k].converged;
+    }
+    warm = true;
+    if (stalled && K && budget > 0) {
+      // a stalled preconditioned solve falls back to the plain method from the current x_o
       K = nullptr;
-      fallback = 1;
+      for (int k = 0; k < nsys; ++k)
+        if (active[k] && !st[k].converged) fallback[k] = 1;
+      for (int k = 0; k < nsys; ++k) active[k] = active[k] && !st[k].converged ? 1 : 0;
       continue;
     }
-    // x_e = Ae^-1 (b_e + kappa D_eo x_o)
-    QCU_TRY(hop(op, ws.t64.ptr(), ws.xo.ptr(), 0, EPI_PLAIN, none, 0, RED_NONE, none, nullptr, nullptr, cfg,
-                lat.stream, false));
-    QCU_TRY(cl_axpy(ws.xe.ptr(), ws.be.ptr(), lat.kappa, ws.t64.ptr(), lat.clover_inv(P64, 0), vh, lat.stream));
-    QCU_TRY(lattice_true_residual(lat, ws.xe.ptr(), ws.xo.ptr(), ws.be.ptr(), ws.bo.ptr(), &res, nullptr,
-                                  ws.r64.ptr()));
+    // x_e = Ae^-1 (b_e + kappa D_eo x_o), then the FP64 true residual of each system
+    QCU_TRY(hop(op, P64, S.t64.ptr(), S.xo.ptr(), 0, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, S.stream,
+                false));
+    QCU_TRY(cl_axpy(op, S.xe.ptr(), S.be.ptr(), 0.0, true, S.t64.ptr(), 1, 0, S.stream));
+    QCU_TRY(residuals_internal(S, S.xe.ptr(), S.xo.ptr(), S.be.ptr(), S.bo.ptr(), S.r64.ptr()));
     trace.mark("x_e + true residual");
-    if (prm.legacy_stop || res <= prm.tol || total_iters >= max_iter || !st.converged) break;
-    target2 *= 0.01;  // tighten the Schur target and continue from the current x_o
+    bool again = false;
+    for (int k = 0; k < nsys; ++k) {
+      const double num = S.host_scal[2 * k], den = S.host_scal[2 * k + 1];
+      res[k] = den > 0 ? sqrt(num / den) : sqrt(num);
+      converged[k] = prm.legacy_stop ? (st[k].converged || converged[k]) : (res[k] <= prm.tol);
+      const bool retry = active[k] && !prm.legacy_stop && !converged[k] && st[k].converged &&
+                         total_iters[k] < max_iter && std::isfinite(res[k]);
+      if (retry) target2[k] *= 0.01;  // tighten the Schur target and continue from x_o
+      active[k] = retry ? 1 : 0;
+      again = again || retry;
+    }
+    if (!again) break;
   }
-  (void)bh2;
-  QCU_TRY(spinor_internal_to_ext(ws.xe.ptr(), lat.g, x, P64, lat.stream));
-  QCU_TRY(spinor_internal_to_ext(ws.xo.ptr(), lat.g, x + half, P64, lat.stream));
-  QCU_HIP(hipStreamSynchronize(lat.stream));
-  auto t1 = std::chrono::steady_clock::now();
+  QCU_TRY(store_systems(S, x_ext, S.xe.ptr(), S.xo.ptr()));
+  QCU_HIP(hipStreamSynchronize(S.stream));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (info) {
-    info->iters = total_iters;
-    info->reliable_updates = total_rel;
-    info->converged = prm.legacy_stop ? st.converged : (res <= prm.tol);
-    info->fallback = fallback;
-    info->true_res = res;
-    info->seconds = std::chrono::duration<double>(t1 - t0).count();
-    info->restarts = restarts;
+    for (int k = 0; k < nsys; ++k) {
+      info[k].iters = total_iters[k];
+      info[k].reliable_updates = total_rel[k];
+      info[k].converged = converged[k];
+      info[k].fallback = fallback[k];
+      info[k].true_res = res[k];
+      info[k].seconds = secs;
+      info[k].restarts = restarts[k];
+    }
   }
   return 0;
 }

@@ -1,10 +1,10 @@
-// tune.hip -- launch-shape autotuner for the hop kernels.
+// tune.hip -- launch-shape autotuner and hop timing.
 //
-// For each (device, lattice shape, precision, clover) the tuner times the hop
-// kernel with HIP events over a small grid of block sizes and XCD remap
-// settings and keeps the fastest.  Results persist in a plain text cache file
-// (one "key block remap ms" line per entry) so that later processes skip the
-// search.
+// For each (device, batch shape, precision) the tuner times the hop kernel
+// with HIP events over a grid of block sizes, XCD-aware block orders and
+// instruction-scheduling groupings, and keeps the fastest.  Results persist in
+// a plain text cache file ("key block remap group" per line) so that later
+// processes skip the search.
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -19,77 +19,123 @@ namespace qcu {
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, LaunchCfg> g_tuned;
-
-std::string key_of(const Lattice &lat, int prec, int clover) {
-  char buf[160];
-  hipDeviceProp_t prop;
-  const char *arch = "gfx";
-  if (hipGetDeviceProperties(&prop, lat.device) == hipSuccess) arch = prop.gcnArchName;
-  snprintf(buf, sizeof(buf), "%s:%dx%dx%dx%d:p%d:c%d", arch, lat.L[0], lat.L[1], lat.L[2], lat.L[3], prec,
-           clover);
-  return std::string(buf);
-}
 }  // namespace
 
-LaunchCfg tuned_cfg(const Lattice &lat, int prec) {
+std::string tune_key(const BatchOp &op, int prec) {
+  std::string k;
+  hipDeviceProp_t prop;
+  int dev = current_device();
+  if (dev >= 0 && hipGetDeviceProperties(&prop, dev) == hipSuccess) k = prop.gcnArchName;
+  char buf[96];
+  for (const SegDesc &d : op.segs) {
+    snprintf(buf, sizeof(buf), ":%dx%dx%dx%dr%d", 2 * d.g.lxh, d.g.ly, d.g.lz, d.g.lt, d.nrhs);
+    k += buf;
+  }
+  snprintf(buf, sizeof(buf), ":p%d:c%d", prec, op.has_clover ? 1 : 0);
+  return k + buf;
+}
+
+LaunchCfg tuned_cfg(const BatchOp &op, int prec) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tuned.find(key_of(lat, prec, lat.has_clover() ? 1 : 0));
+  auto it = g_tuned.find(tune_key(op, prec));
   if (it != g_tuned.end()) return it->second;
   return LaunchCfg();
 }
 
-}  // namespace qcu
+// average milliseconds per launch of the hop kernel on resident fields
+int bench_hop(const BatchOp &op, hipStream_t s, int prec, int clover, int xpay, int iters, const LaunchCfg &cfg,
+              double *ms) {
+  QCU_CHECK(prec >= 0 && prec <= 2 && iters > 0, QCU_ERR_ARG, "bad precision or iteration count");
+  for (const Lattice *l : op.lats)
+    QCU_CHECK((l->prec_mask >> prec) & 1, QCU_ERR_STATE, "links of the requested precision are not resident");
+  const int n = (int)op.nvsites;
+  Spinor a, b, c, g64;
+  QCU_TRY(a.create(prec, n));
+  QCU_TRY(b.create(prec, n));
+  QCU_TRY(c.create(prec, n));
+  QCU_TRY(g64.create(P64, n));
+  QCU_TRY(gen_gauss(g64.data.p, (long)n * 12, 77, s));
+  QCU_TRY(spinor_convert(op, a.ptr(), prec, g64.ptr(), P64, s));
+  QCU_TRY(spinor_convert(op, c.ptr(), prec, g64.ptr(), P64, s));
+  SpinorPtr none{};
+  auto launch = [&](int k) {
+    return hop(op, prec, (k & 1) ? a.ptr() : b.ptr(), (k & 1) ? b.ptr() : a.ptr(), k & 1, xpay ? EPI_XPAY : EPI_PLAIN,
+               c.ptr(), RED_NONE, none, nullptr, nullptr, cfg, s, clover != 0);
+  };
+  for (int k = 0; k < 3; ++k) QCU_TRY(launch(k));
+  hipEvent_t e0, e1;
+  QCU_HIP(hipEventCreate(&e0));
+  QCU_HIP(hipEventCreate(&e1));
+  QCU_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < iters; ++k) QCU_TRY(launch(k));
+  QCU_HIP(hipEventRecord(e1, s));
+  QCU_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  QCU_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / iters;
+  return 0;
+}
 
-using namespace qcu;
-
-extern "C" {
-
-int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap, double *ms);
-
-// Sweeps launch shapes of the hop kernel (the clover epilogue when the lattice
-// has one) for one precision and records the fastest.  best_ms may be null.
-int qcuTune(QcuLatticeHandle h, int prec, double *best_ms) {
-  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
-  Lattice &lat = h->lat;
-  const int clover = lat.has_clover() ? 1 : 0;
+static int tune_op(const BatchOp &op, hipStream_t s, int prec, double *best_ms, LaunchCfg *out) {
+  const int clover = op.has_clover ? 1 : 0;
   const int blocks[] = {64, 128, 256};
   LaunchCfg best;
   double bms = 1e30;
   for (int grp = 0; grp < 3; ++grp)
     for (int b : blocks)
       for (int r = 0; r < 2; ++r) {
-        {
-          std::lock_guard<std::mutex> lk(g_tune_mu);
-          LaunchCfg c;
-          c.block = b;
-          c.remap = r;
-          c.group = grp;
-          g_tuned[key_of(lat, prec, clover)] = c;
-        }
+        LaunchCfg c;
+        c.block = b;
+        c.remap = r;
+        c.group = grp;
         double ms = 0;
-        QCU_TRY(qcuBenchHop(h, prec, clover, 1, 20, b, r, &ms));
+        QCU_TRY(bench_hop(op, s, prec, clover, 1, 20, c, &ms));
         if (ms < bms) {
           bms = ms;
-          best.block = b;
-          best.remap = r;
-          best.group = grp;
+          best = c;
         }
       }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[key_of(lat, prec, clover)] = best;
+    g_tuned[tune_key(op, prec)] = best;
   }
   if (best_ms) *best_ms = bms;
+  if (out) *out = best;
   return 0;
+}
+
+}  // namespace qcu
+
+using namespace qcu;
+
+struct QcuBatch_s;
+namespace qcu {
+const BatchOp &batch_op(const QcuBatch_s *b);
+hipStream_t batch_stream(const QcuBatch_s *b);
+}  // namespace qcu
+
+extern "C" {
+
+int qcuTune(QcuLatticeHandle h, int prec, double *best_ms) {
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_CHECK(h->lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
+  return tune_op(h->lat.self, h->lat.stream, prec, best_ms, nullptr);
 }
 
 int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group) {
   QCU_CHECK(h && block && remap && group, QCU_ERR_ARG, "null argument");
-  LaunchCfg c = tuned_cfg(h->lat, prec);
+  LaunchCfg c = tuned_cfg(h->lat.self, prec);
   *block = c.block;
   *remap = c.remap;
   *group = c.group;
   return 0;
+}
+
+int qcuBatchTune(QcuBatchHandle h, int prec, double *best_ms) {
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  return tune_op(batch_op(h), batch_stream(h), prec, best_ms, nullptr);
 }
 
 int qcuTuneSave(const char *path) {
@@ -107,10 +153,10 @@ int qcuTuneLoad(const char *path) {
   QCU_CHECK(path, QCU_ERR_ARG, "null path");
   FILE *f = fopen(path, "r");
   if (!f) return 0;  // no cache yet
-  char key[160];
+  char key[512];
   int b, r, grp;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  while (fscanf(f, "%159s %d %d %d", key, &b, &r, &grp) == 4) {
+  while (fscanf(f, "%511s %d %d %d", key, &b, &r, &grp) == 4) {
     if (b >= 64 && b <= 256 && b % 64 == 0 && (r == 0 || r == 1) && grp >= 0 && grp <= 2) {
       LaunchCfg c;
       c.block = b;

@@ -153,6 +153,65 @@ class Lattice:
         return {"block": b.value, "remap": r.value, "group": g.value, "ms": ms.value}
 
 
+class Batch:
+    """Many systems solved together: nrhs[k] right-hand sides on lattices[k].
+
+    Every kernel launch covers all systems (one launch per kernel for the
+    whole batch; the right-hand sides of one lattice share link loads).  The
+    lattices must stay alive and keep their gauge fields while the batch is
+    used.  System order: lattice 0 rhs 0.., lattice 1 rhs 0.., ...
+    """
+
+    def __init__(self, lattices, nrhs):
+        self._lib = _lib.load()
+        self.lattices = list(lattices)
+        self.nrhs = [int(n) for n in nrhs]
+        hs = (C.c_void_p * len(self.lattices))(*[l.h.value for l in self.lattices])
+        nr = (C.c_int * len(self.nrhs))(*self.nrhs)
+        h = C.c_void_p()
+        _lib.check(self._lib.qcuBatchCreate(C.byref(h), hs, nr, len(self.lattices)))
+        self.h = h
+        n = C.c_int()
+        _lib.check(self._lib.qcuBatchSystems(self.h, C.byref(n)))
+        self.nsys = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.qcuBatchDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, bs, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
+              use_graph=True, xs=None):
+        assert len(bs) == self.nsys
+        if xs is None:
+            xs = [torch.empty_like(b) for b in bs]
+        bp = (C.c_void_p * self.nsys)(*[_ptr(b).value for b in bs])
+        xp = (C.c_void_p * self.nsys)(*[_ptr(x).value for x in xs])
+        p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec), reliable_delta=reliable_delta,
+                       chunk=chunk, use_graph=int(use_graph), precond=None, legacy_stop=0)
+        info = (SolveInfo * self.nsys)()
+        _lib.check(self._lib.qcuBatchSolve(self.h, xp, bp, C.byref(p), info))
+        res = [SolveResult(i.iters, i.reliable_updates, bool(i.converged), bool(i.fallback), i.true_res,
+                           i.seconds, i.restarts) for i in info]
+        return xs, res
+
+    def bench_hop(self, prec=FP64, xpay=True, iters=50):
+        ms = C.c_double()
+        _lib.check(self._lib.qcuBatchBenchHop(self.h, int(prec), int(xpay), int(iters), C.byref(ms)))
+        return ms.value
+
+    def tune(self, prec=FP64):
+        ms = C.c_double()
+        _lib.check(self._lib.qcuBatchTune(self.h, int(prec), C.byref(ms)))
+        return ms.value
+
+
 def tune_save(path):
     _lib.check(_lib.load().qcuTuneSave(str(path).encode()))
 
