@@ -146,6 +146,44 @@ int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group)
 int qcuTuneSave(const char *path);
 int qcuTuneLoad(const char *path);
 
+/* ---- learned preconditioner (new) ----------------------------------------
+ * A gauge-covariant linear network K on the lattice h: nlayers layers mapping
+ * chan[l] -> chan[l+1] spinor channels (chan[0] = chan[nlayers] = 1, at most 8),
+ *   out_o(x) = sum_i sum_mu [A_mu U_mu(x) in_i(x+mu) + B_mu U_mu(x-mu)^dag in_i(x-mu)] + C in_i(x)
+ * with learned 4x4 complex spin matrices per (layer, term, o, i), FP32.
+ * Weight order: layer, term t (2 mu: forward hop A_mu, 2 mu + 1: backward
+ * hop B_mu, 8: C), output channel, input channel, spin row, spin column.
+ * Initialised to the identity plus seeded noise of size init_scale.
+ * Passed as QcuSolveParam.precond it is applied as a fixed right
+ * preconditioner of the odd-site Schur system (odd input, even sites zero,
+ * odd part of the output kept). */
+typedef struct QcuNet_s *QcuNetHandle;
+int qcuNetCreate(QcuNetHandle *h, QcuLatticeHandle lat, int nlayers, const int *chan, uint64_t seed,
+                 double init_scale);
+int qcuNetDestroy(QcuNetHandle h);
+long qcuNetWeightCount(QcuNetHandle h);
+/* complex128 weights / gradient on the HOST, n = qcuNetWeightCount */
+int qcuNetGetWeights(QcuNetHandle h, double *host, long n);
+int qcuNetSetWeights(QcuNetHandle h, const double *host, long n);
+int qcuNetGetGradient(QcuNetHandle h, double *host, long n);
+/* full-lattice fields [2][Vh][4][3] complex128 (device): out = K in */
+int qcuNetForward(QcuNetHandle h, void *out, const void *in);
+/* gradient of a real loss L with respect to the weights (read back with
+ * qcuNetGetGradient) and, when grad_in is not NULL, to the input, given
+ * grad_out = dL/dRe(out) + i dL/dIm(out) at the input `in` */
+int qcuNetBackward(QcuNetHandle h, void *grad_in, const void *grad_out, const void *in);
+/* Adam training on odd-site sources srcs[k] (one-parity external fields,
+ * device) of the loss |Mhat K b - b|^2 / |b|^2, Mhat the FP32 odd Schur
+ * operator; losses[e] = mean loss seen during epoch e (may be NULL) */
+int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int batch, double lr,
+                double *losses, double *seconds);
+/* loss of one source and its weight gradient (read with qcuNetGetGradient) */
+int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss);
+/* mean loss over the sources, no update */
+int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss);
+/* out = K in on one-parity (odd) external fields, as applied inside the solver */
+int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);
+
 #ifdef __cplusplus
 }
 #endif

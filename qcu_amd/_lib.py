@@ -81,6 +81,20 @@ SIGNATURES = {
                              C.POINTER(C.c_int)]),
     "qcuTuneSave": (C.c_int, [C.c_char_p]),
     "qcuTuneLoad": (C.c_int, [C.c_char_p]),
+    # learned preconditioner
+    "qcuNetCreate": (C.c_int, [C.POINTER(_VP), _VP, C.c_int, C.POINTER(C.c_int), C.c_uint64, C.c_double]),
+    "qcuNetDestroy": (C.c_int, [_VP]),
+    "qcuNetWeightCount": (C.c_long, [_VP]),
+    "qcuNetGetWeights": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_long]),
+    "qcuNetSetWeights": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_long]),
+    "qcuNetGetGradient": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_long]),
+    "qcuNetForward": (C.c_int, [_VP, _VP, _VP]),
+    "qcuNetBackward": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "qcuNetTrain": (C.c_int, [_VP, C.POINTER(_VP), C.c_int, C.c_int, C.c_int, C.c_double,
+                              C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "qcuNetLoss": (C.c_int, [_VP, C.POINTER(_VP), C.c_int, C.POINTER(C.c_double)]),
+    "qcuNetApplyOdd": (C.c_int, [_VP, _VP, _VP]),
+    "qcuNetLossGrad": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
 }
 
 
@@ -90,7 +104,7 @@ def header_symbols():
     path = os.path.join(os.path.dirname(HERE), "include", "qcu.h")
     txt = open(path).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:void|int|const char \*)\s*\*?\s*(\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(?:void|int|long|const char \*)\s*\*?\s*(\w+)\s*\(", txt)))
 
 
 def load():

@@ -33,7 +33,7 @@ struct QcuBatch_s {
   }
 };
 
-namespace {
+namespace qcu {
 
 int check_dev_ptr(const void *p, const char *what) {
   if (!p) {
@@ -53,6 +53,10 @@ int check_dev_ptr(const void *p, const char *what) {
   }
   return 0;
 }
+
+}  // namespace qcu
+
+namespace {
 
 int check_param(const QcuParam *param) {
   QCU_CHECK(param != nullptr, QCU_ERR_ARG, "param is null");
@@ -324,7 +328,13 @@ int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p,
   void *xs[1] = {x};
   const void *bs[1] = {b};
   QcuSolveInfo tmp{};
-  return full_solve(*S, xs, bs, *p, info ? info : &tmp);
+  QcuSolveParam q = *p;
+  if (p->precond) {
+    Precond *K = nullptr;
+    QCU_TRY(net_precond(p->precond, &lat, &K));
+    q.precond = K;
+  }
+  return full_solve(*S, xs, bs, q, info ? info : &tmp);
 }
 
 int qcuBenchHop(QcuLatticeHandle h, int prec, int clover, int xpay, int iters, int block, int remap, double *ms) {

@@ -1,0 +1,950 @@
+// learned.hip -- learned gauge-covariant linear preconditioner (new; no
+// counterpart in the reference).
+//
+// Network: L layers on the full lattice, layer l maps chan[l] -> chan[l+1]
+// channels of spinor fields:
+//   out_o(x) = sum_i [ sum_mu ( A^{oi}_mu U_mu(x) in_i(x+mu) + B^{oi}_mu U_mu(x-mu)^dag in_i(x-mu) )
+//                      + C^{oi} in_i(x) ]
+// with learned 4x4 complex spin matrices A, B, C (acting on each colour) and a
+// -1 phase on hops across the time boundary for antiperiodic lattices.  Terms
+// t = 2 mu (forward hop), 2 mu + 1 (backward hop), 8 (site).  Everything is
+// FP32.  Weights are W[l][t][o][i][4x4] complex.
+//
+// As a right preconditioner of the odd-site Schur operator the input (an odd
+// vector) is embedded in the full lattice with zero even sites, and the odd
+// part of the single output channel is kept.
+//
+// Kernels: layer forward (one thread per site and group of output channels,
+// weights wavefront-uniform so they come through scalar loads), the data
+// backward is the forward kernel with conjugate-transposed weights and
+// swapped hop directions, the weight gradient is a two-stage deterministic
+// reduction of g_o(x) (T_t a_i)(x)^dag over sites and colours, and Adam runs on
+// the device.  Training minimises |Mhat K b - b|^2 / |b|^2.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/qcu.h"
+#include "qcu_batch.h"
+#include "qcu_lattice.h"
+#include "qcu_reduce.h"
+#include "qcu_solver.h"
+
+namespace qcu {
+
+typedef cplx<float> cf;
+
+constexpr int kNetMaxLayers = 8;
+constexpr int kNetMaxChan = 8;
+constexpr int kWgradBlocks = 120;  // site chunks per weight-gradient combination
+
+struct NetLayerArgs {
+  SpinorPtr in;   // channel c, parity p at in.p + (c * 2 + p) * vh * 96 bytes, stride vh
+  SpinorPtr out;  // same layout
+  const float2 *W;
+  LinkPtr U;      // FP32 links
+  Geom g;
+  int cin, cout;
+  int parity0;    // first output parity (the grid's y index adds to it)
+  const SysState *st;
+};
+
+QCU_D SpinorPtr chan_field(const SpinorPtr &base, int c, int p, int vh) {
+  SpinorPtr f;
+  f.p = (char *)base.p + ((size_t)c * 2 + p) * (size_t)vh * 96;
+  f.norm = nullptr;
+  f.stride = vh;
+  return f;
+}
+
+// neighbour site (half index) and link for term t at output site i of parity p
+struct TermGeo {
+  int j;      // input site (half index)
+  int jp;     // input parity
+  int hop;    // 0: site term
+  int fwd;
+  int slab;   // link slab (parity * 4 + mu)
+  int li;     // link site
+  float ph;
+};
+
+QCU_D TermGeo term_geo(const Geom &g, int i, int p, int t) {
+  TermGeo r;
+  r.j = i;
+  r.jp = p;
+  r.hop = t < 8;
+  r.fwd = !(t & 1);
+  r.slab = 0;
+  r.li = i;
+  r.ph = 1.f;
+  if (!r.hop) return r;
+  const int mu = t >> 1;
+  const Site s = decode(g, i, p);
+  const int q = 1 - p;
+  const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
+  int j = i;
+  if (mu == 0) {
+    const int xe = 1 - s.xodd;
+    j = r.fwd ? i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh : i - xe + xe * (int)(s.xh == 0) * g.lxh;
+  } else if (mu == 1) {
+    j = r.fwd ? i + sy - (int)(s.y == g.ly - 1) * g.ly * sy : i - sy + (int)(s.y == 0) * g.ly * sy;
+  } else if (mu == 2) {
+    j = r.fwd ? i + sz - (int)(s.z == g.lz - 1) * g.lz * sz : i - sz + (int)(s.z == 0) * g.lz * sz;
+  } else {
+    j = r.fwd ? i + st_ - (int)(s.t == g.lt - 1) * g.lt * st_ : i - st_ + (int)(s.t == 0) * g.lt * st_;
+    const bool cross = r.fwd ? (s.t == g.lt - 1) : (s.t == 0);
+    r.ph = (cross && g.tbc < 0) ? -1.f : 1.f;
+  }
+  r.j = j;
+  r.jp = q;
+  r.slab = r.fwd ? p * 4 + mu : q * 4 + mu;
+  r.li = r.fwd ? i : j;
+  return r;
+}
+
+// v <- ph * U v (fwd) or ph * U^dag v, on each of the four spins
+QCU_D void transport4(const cf (&u)[9], cf (&v)[12], int fwd, float ph) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    cf a[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      cf acc = mk(0.f, 0.f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (fwd) cmad(acc, u[r * 3 + c], v[s * 3 + c]);
+        else cmadj(acc, u[c * 3 + r], v[s * 3 + c]);
+      }
+      a[r] = mk(ph * acc.re, ph * acc.im);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) v[s * 3 + r] = a[r];
+  }
+}
+
+// grid: x sites, y output parity, z output-channel group of CO channels
+template <int CO>
+__global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
+  if (a.st && a.st->done) return;
+  const Geom g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = a.parity0 + blockIdx.y;
+  const int og = blockIdx.z * CO;
+  if (i >= g.vh) return;
+  cf acc[CO][12];
+#pragma unroll
+  for (int o = 0; o < CO; ++o)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
+  const int cin = a.cin, cout = a.cout;
+#pragma unroll 1
+  for (int t = 0; t < 9; ++t) {
+    const TermGeo tg = term_geo(g, i, p, t);
+    cf u[9];
+    if (tg.hop) LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+#pragma unroll 1
+    for (int ci = 0; ci < cin; ++ci) {
+      cf v[12];
+      SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
+      if (tg.hop) transport4(u, v, tg.fwd, tg.ph);
+#pragma unroll
+      for (int o = 0; o < CO; ++o) {
+        if (og + o >= cout) break;
+        const float2 *w = a.W + (((size_t)t * cout + og + o) * cin + ci) * 16;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int s2 = 0; s2 < 4; ++s2) {
+            const float2 ww = w[s * 4 + s2];
+            const cf wc = mk(ww.x, ww.y);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cmad(acc[o][s * 3 + c], wc, v[s2 * 3 + c]);
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < CO; ++o) {
+    if (og + o >= cout) break;
+    SpinorIO<P32>::store(chan_field(a.out, og + o, p, g.vh), i, acc[o]);
+  }
+}
+
+// weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
+//   sum over sites of both parities and colours of g_o(x)[s][c] conj((T_t a_ci)(x)[s2][c])
+struct NetWgradArgs {
+  SpinorPtr act;   // layer input channels
+  SpinorPtr gout;  // gradient w.r.t. the layer output channels
+  LinkPtr U;
+  Geom g;
+  int cin, cout;
+  float *partials;  // [combo][block][CO * 32]
+};
+
+template <int CO>
+__global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
+  const Geom g = a.g;
+  const int t = blockIdx.y / a.cin, ci = blockIdx.y % a.cin;
+  const int og = blockIdx.z * CO;
+  float acc[CO][32];
+#pragma unroll
+  for (int o = 0; o < CO; ++o)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[o][k] = 0.f;
+  const int nsite = 2 * g.vh;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nsite; k += gridDim.x * blockDim.x) {
+    const int p = k >= g.vh ? 1 : 0;
+    const int i = k - p * g.vh;
+    const TermGeo tg = term_geo(g, i, p, t);
+    cf v[12];
+    SpinorIO<P32>::load(chan_field(a.act, ci, tg.jp, g.vh), tg.j, v);
+    if (tg.hop) {
+      cf u[9];
+      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+      transport4(u, v, tg.fwd, tg.ph);
+    }
+#pragma unroll
+    for (int o = 0; o < CO; ++o) {
+      if (og + o >= a.cout) break;
+      cf gg[12];
+      SpinorIO<P32>::load(chan_field(a.gout, og + o, p, g.vh), i, gg);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) {
+          cf e = mk(0.f, 0.f);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) cmadj(e, v[s2 * 3 + c], gg[s * 3 + c]);  // g * conj(v)
+          acc[o][2 * (s * 4 + s2)] += e.re;
+          acc[o][2 * (s * 4 + s2) + 1] += e.im;
+        }
+    }
+  }
+  // block reduction in a fixed order: waves by shuffles, then LDS across waves
+  __shared__ float sh[4][CO * 32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 0; o < CO; ++o)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      float v = acc[o][k];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) sh[wave][o * 32 + k] = v;
+    }
+  __syncthreads();
+  const int combo = blockIdx.z * gridDim.y + blockIdx.y;
+  float *dst = a.partials + ((size_t)combo * gridDim.x + blockIdx.x) * (CO * 32);
+  for (int k = threadIdx.x; k < CO * 32; k += blockDim.x) {
+    float s = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w][k];
+    dst[k] = s;
+  }
+}
+
+// second stage: gW[t][o][ci][16] = sum over blocks (FP64 accumulation)
+__global__ void k_net_wgrad_sum(const float *partials, int nblk, int cin, int cout, int co, int ngroups,
+                                float2 *gW, int accumulate) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 9 * cout * cin * 16
+  const int total = 9 * cout * cin * 16;
+  if (idx >= total) return;
+  const int e = idx % 16;
+  const int ci = (idx / 16) % cin;
+  const int o = (idx / 16 / cin) % cout;
+  const int t = idx / 16 / cin / cout;
+  const int gz = o / co, ol = o % co;
+  const int combo = gz * (9 * cin) + t * cin + ci;
+  double re = 0, im = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const float *src = partials + ((size_t)combo * nblk + b) * (co * 32) + ol * 32 + 2 * e;
+    re += src[0];
+    im += src[1];
+  }
+  float2 v = make_float2((float)re, (float)im);
+  if (accumulate) {
+    v.x += gW[idx].x;
+    v.y += gW[idx].y;
+  }
+  gW[idx] = v;
+}
+
+// W'[t'][i][o] = (W[t][o][i])^dag with t' the adjoint term (hops swap direction)
+__global__ void k_net_transpose(const float2 *W, float2 *Wt, int cin, int cout) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = 9 * cout * cin * 16;
+  if (idx >= total) return;
+  const int e = idx % 16, s = e / 4, s2 = e % 4;
+  const int ci = (idx / 16) % cin;
+  const int o = (idx / 16 / cin) % cout;
+  const int t = idx / 16 / cin / cout;
+  const int ta = t == 8 ? 8 : (t ^ 1);
+  const float2 w = W[idx];
+  // Wt has cin' = cout, cout' = cin: index [ta][ci][o][s2][s]
+  Wt[(((size_t)ta * cin + ci) * cout + o) * 16 + s2 * 4 + s] = make_float2(w.x, -w.y);
+}
+
+// embed an odd-site vector (precision P) as channel 0 of a full FP32 field
+template <int P>
+__global__ void k_net_embed(SpinorPtr full, SpinorPtr odd, int vh, const SysState *st) {
+  if (st && st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  typedef typename real_of<P>::type T;
+  cplx<T> v[12];
+  SpinorIO<P>::load(odd, i, v);
+  cf w[12], z[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    w[k] = mk((float)v[k].re, (float)v[k].im);
+    z[k] = mk(0.f, 0.f);
+  }
+  SpinorIO<P32>::store(chan_field(full, 0, 1, vh), i, w);
+  SpinorIO<P32>::store(chan_field(full, 0, 0, vh), i, z);
+}
+
+template <int P>
+__global__ void k_net_extract(SpinorPtr odd, SpinorPtr full, int vh, const SysState *st) {
+  if (st && st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  typedef typename real_of<P>::type T;
+  cf w[12];
+  SpinorIO<P32>::load(chan_field(full, 0, 1, vh), i, w);
+  cplx<T> v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = mk<T>((T)w[k].re, (T)w[k].im);
+  SpinorIO<P>::store(odd, i, v);
+}
+
+// z = gamma5 Ao^-1 r (FP32, odd sites); gamma5 = diag(1, 1, -1, -1) in this basis
+__global__ void k_g5_clover(SpinorPtr z, SpinorPtr r, CloverPtr C, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cf v[12];
+  SpinorIO<P32>::load(r, i, v);
+  if (C.p) {
+    float cb[36];
+    clover_load_block(C, i, 0, cb);
+    clover_apply_block(cb, v, 0);
+    clover_load_block(C, i, 1, cb);
+    clover_apply_block(cb, v, 1);
+  }
+#pragma unroll
+  for (int k = 6; k < 12; ++k) v[k] = mk(-v[k].re, -v[k].im);
+  SpinorIO<P32>::store(z, i, v);
+}
+
+// out = a * (r - k2 gamma5 u)
+__global__ void k_g5_xpay(SpinorPtr out, SpinorPtr r, SpinorPtr u, float k2, float a, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cf rv[12], uv[12];
+  SpinorIO<P32>::load(r, i, rv);
+  SpinorIO<P32>::load(u, i, uv);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const float sg = k < 6 ? 1.f : -1.f;
+    rv[k] = mk(a * (rv[k].re - k2 * sg * uv[k].re), a * (rv[k].im - k2 * sg * uv[k].im));
+  }
+  SpinorIO<P32>::store(out, i, rv);
+}
+
+// res = y - b (FP32 odd), |res|^2 into partials per block (FP64)
+__global__ void k_sub_norm(SpinorPtr res, SpinorPtr y, SpinorPtr b, float bscale, int vh, double *partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0};
+  if (i < vh) {
+    cf yv[12], bv[12];
+    SpinorIO<P32>::load(y, i, yv);
+    SpinorIO<P32>::load(b, i, bv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      yv[k] = mk(yv[k].re - bscale * bv[k].re, yv[k].im - bscale * bv[k].im);
+      acc[0] += (double)yv[k].re * yv[k].re + (double)yv[k].im * yv[k].im;
+    }
+    SpinorIO<P32>::store(res, i, yv);
+  }
+  block_partial<1>(acc, partials, blockIdx.x);
+}
+
+// FP32 odd vector <- external one-parity complex128 field, scaled
+__global__ void k_load_odd(SpinorPtr dst, const double2 *ext, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cf v[12];
+  for (int k = 0; k < 12; ++k) {
+    const double2 a = ext[(size_t)i * 12 + k];
+    v[k] = mk((float)a.x, (float)a.y);
+  }
+  SpinorIO<P32>::store(dst, i, v);
+}
+
+__global__ void k_store_odd(double2 *ext, SpinorPtr src, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cf v[12];
+  SpinorIO<P32>::load(src, i, v);
+  for (int k = 0; k < 12; ++k) ext[(size_t)i * 12 + k] = make_double2(v[k].re, v[k].im);
+}
+
+// external [C][2][Vh][12] complex128 <-> internal channel fields (both parities)
+__global__ void k_full_in(SpinorPtr base, const double2 *ext, int nch, int vh) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nch * 2 * vh) return;
+  const int i = (int)(idx % vh), p = (int)((idx / vh) % 2), c = (int)(idx / vh / 2);
+  cf v[12];
+  for (int k = 0; k < 12; ++k) {
+    const double2 a = ext[((size_t)(c * 2 + p) * vh + i) * 12 + k];
+    v[k] = mk((float)a.x, (float)a.y);
+  }
+  SpinorIO<P32>::store(chan_field(base, c, p, vh), i, v);
+}
+
+__global__ void k_full_out(double2 *ext, SpinorPtr base, int nch, int vh) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nch * 2 * vh) return;
+  const int i = (int)(idx % vh), p = (int)((idx / vh) % 2), c = (int)(idx / vh / 2);
+  cf v[12];
+  SpinorIO<P32>::load(chan_field(base, c, p, vh), i, v);
+  for (int k = 0; k < 12; ++k) ext[((size_t)(c * 2 + p) * vh + i) * 12 + k] = make_double2(v[k].re, v[k].im);
+}
+
+__global__ void k_add_f32(float *acc, const float *x, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) acc[k] += x[k];
+}
+
+// Adam on interleaved (re, im) FP32 parameters
+__global__ void k_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1, float b2, float c1,
+                       float c2, float eps, float gscale) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float gg = g[k] * gscale;
+  const float mm = b1 * m[k] + (1.f - b1) * gg;
+  const float vv = b2 * v[k] + (1.f - b2) * gg * gg;
+  m[k] = mm;
+  v[k] = vv;
+  w[k] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// ---------------------------------------------------------------- host side
+struct Net : Precond {
+  Lattice *lat = nullptr;
+  int nl = 0;
+  int chan[kNetMaxLayers + 1] = {0};
+  std::vector<size_t> woff;  // float2 offset of each layer's weights
+  size_t nw = 0;             // complex weights
+  DevBuf W, Wt, gW, m, v, wpart;
+  DevBuf act;                // layer inputs: layer l has chan[l] channel fields
+  std::vector<size_t> aoff;  // byte offset of layer l's input block
+  DevBuf gbuf[2];            // gradient ping-pong (max channels)
+  DevBuf tmp;                // odd FP32 vectors for training
+  DevBuf red;                // reduction partials
+  int adam_t = 0;
+  size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
+
+  int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
+  SpinorPtr act_ptr(int layer) const {
+    SpinorPtr s;
+    s.p = (char *)act.p + aoff[layer];
+    s.norm = nullptr;
+    s.stride = lat->g.vh;
+    return s;
+  }
+  int layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
+            const SysState *st, hipStream_t s);
+  int forward(const SysState *st, hipStream_t s, bool odd_only_last);
+  int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in);
+  int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
+};
+
+static size_t ndiv(size_t a, size_t b) { return (a + b - 1) / b; }
+
+int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale) {
+  QCU_CHECK(l && l->gauge_loaded, QCU_ERR_STATE, "net: lattice has no gauge field");
+  QCU_CHECK(nlayers >= 1 && nlayers <= kNetMaxLayers, QCU_ERR_ARG, "net: 1 to 8 layers");
+  QCU_CHECK((l->prec_mask >> P32) & 1, QCU_ERR_STATE, "net: FP32 links not resident");
+  lat = l;
+  nl = nlayers;
+  for (int k = 0; k <= nl; ++k) {
+    QCU_CHECK(ch[k] >= 1 && ch[k] <= kNetMaxChan, QCU_ERR_ARG, "net: 1 to 8 channels per layer");
+    chan[k] = ch[k];
+  }
+  QCU_CHECK(chan[0] == 1 && chan[nl] == 1, QCU_ERR_ARG, "net: one input and one output channel");
+  woff.assign(nl + 1, 0);
+  for (int k = 0; k < nl; ++k) woff[k + 1] = woff[k] + (size_t)9 * chan[k] * chan[k + 1] * 16;
+  nw = woff[nl];
+  QCU_TRY(W.alloc(nw * 8));
+  QCU_TRY(Wt.alloc(nw * 8));
+  QCU_TRY(gW.alloc(nw * 8));
+  QCU_TRY(m.alloc(nw * 8));
+  QCU_TRY(v.alloc(nw * 8));
+  aoff.assign(nl + 1, 0);
+  size_t off = 0;
+  for (int k = 0; k <= nl; ++k) {
+    aoff[k] = off;
+    off += chan[k] * field_bytes();
+  }
+  QCU_TRY(act.alloc(off));
+  int cmax = 1;
+  for (int k = 0; k <= nl; ++k) cmax = std::max(cmax, chan[k]);
+  QCU_TRY(gbuf[0].alloc(cmax * field_bytes()));
+  QCU_TRY(gbuf[1].alloc(cmax * field_bytes()));
+  QCU_TRY(tmp.alloc((size_t)6 * lat->g.vh * 96));
+  QCU_TRY(red.alloc(((size_t)lat->g.vh / 64 + 16) * 8));
+  // weight-gradient partials: combos (9 cin * groups) * blocks * CO*32 floats
+  size_t wp = 0;
+  for (int k = 0; k < nl; ++k) {
+    const int co = chan[k + 1] >= 2 ? 2 : 1;
+    wp = std::max(wp, (size_t)9 * chan[k] * ndiv(chan[k + 1], co) * kWgradBlocks * co * 32);
+  }
+  QCU_TRY(wpart.alloc(wp * 4));
+  // initialisation: the identity through channel 0 plus small seeded noise
+  std::vector<float> h(nw * 2, 0.f);
+  uint64_t z = seed * 0x9e3779b97f4a7c15ULL + 12345;
+  auto rnd = [&]() {
+    z ^= z << 13;
+    z ^= z >> 7;
+    z ^= z << 17;
+    return ((double)(z >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+  };
+  for (int k = 0; k < nl; ++k) {
+    const int ci = chan[k], co = chan[k + 1];
+    const double sc = init_scale / sqrt(9.0 * ci * 4.0);
+    for (size_t e = woff[k]; e < woff[k + 1]; ++e) {
+      h[2 * e] = (float)(sc * rnd());
+      h[2 * e + 1] = (float)(sc * rnd());
+    }
+    // site term, output 0 <- input 0: unit spin matrix
+    for (int s = 0; s < 4; ++s) h[2 * (woff[k] + (((size_t)8 * co + 0) * ci + 0) * 16 + s * 5)] += 1.f;
+  }
+  QCU_HIP(hipMemcpy(W.p, h.data(), nw * 8, hipMemcpyHostToDevice));
+  QCU_HIP(hipMemset(m.p, 0, nw * 8));
+  QCU_HIP(hipMemset(v.p, 0, nw * 8));
+  adam_t = 0;
+  return 0;
+}
+
+int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
+               const SysState *st, hipStream_t s) {
+  NetLayerArgs a;
+  a.in = in;
+  a.out = out;
+  a.W = Wl;
+  a.U = lat->links(P32);
+  a.g = lat->g;
+  a.cin = cin;
+  a.cout = cout;
+  a.parity0 = parity0;
+  a.st = st;
+  const int bs = 128;
+  const int co = cout >= 4 ? 4 : cout;
+  dim3 grid((unsigned)ndiv(lat->g.vh, bs), nparity, (unsigned)ndiv(cout, co));
+  switch (co) {
+    case 1: hipLaunchKernelGGL(k_net_layer<1>, grid, dim3(bs), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_net_layer<2>, grid, dim3(bs), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_net_layer<3>, grid, dim3(bs), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_net_layer<4>, grid, dim3(bs), 0, s, a); break;
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+// act[0] holds the input; fills act[1..nl] (act[nl] = output)
+int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last) {
+  for (int k = 0; k < nl; ++k) {
+    const bool last = k == nl - 1;
+    const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
+    QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s));
+  }
+  return 0;
+}
+
+// gout_last: gradient w.r.t. act[nl]; accumulates nothing, writes gW (this sample)
+int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr grad_in) {
+  // conjugate-transposed weights for the data gradient
+  for (int k = 0; k < nl; ++k) {
+    const int total = 9 * chan[k] * chan[k + 1] * 16;
+    hipLaunchKernelGGL(k_net_transpose, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s,
+                       (const float2 *)W.p + woff[k], (float2 *)Wt.p + woff[k], chan[k], chan[k + 1]);
+  }
+  QCU_HIP(hipGetLastError());
+  SpinorPtr gcur = gout;
+  int ping = 0;
+  for (int k = nl - 1; k >= 0; --k) {
+    const int cin = chan[k], cout = chan[k + 1];
+    // weight gradient of layer k
+    NetWgradArgs a;
+    a.act = act_ptr(k);
+    a.gout = gcur;
+    a.U = lat->links(P32);
+    a.g = lat->g;
+    a.cin = cin;
+    a.cout = cout;
+    a.partials = (float *)wpart.p;
+    const int co = cout >= 2 ? 2 : 1;
+    const int ng = (int)ndiv(cout, co);
+    dim3 grid(kWgradBlocks, 9 * cin, ng);
+    if (co == 2) hipLaunchKernelGGL(k_net_wgrad<2>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_net_wgrad<1>, grid, dim3(256), 0, s, a);
+    QCU_HIP(hipGetLastError());
+    const int total = 9 * cout * cin * 16;
+    hipLaunchKernelGGL(k_net_wgrad_sum, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, (const float *)wpart.p,
+                       kWgradBlocks, cin, cout, co, ng, (float2 *)gW.p + woff[k], 0);
+    QCU_HIP(hipGetLastError());
+    if (k == 0 && !want_grad_in) break;
+    // data gradient: the forward layer with the transposed weights (cin <-> cout)
+    SpinorPtr gnext;
+    if (k == 0) {
+      gnext = grad_in;
+    } else {
+      gnext.p = gbuf[ping].p;
+      gnext.norm = nullptr;
+      gnext.stride = lat->g.vh;
+    }
+    QCU_TRY(layer(gcur, gnext, (const float2 *)Wt.p + woff[k], cout, cin, 0, 2, nullptr, s));
+    gcur = gnext;
+    ping ^= 1;
+  }
+  return 0;
+}
+
+int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
+  const int vh = lat->g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  const SpinorPtr a0 = act_ptr(0);
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_net_embed<P64>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
+    case P32: hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
+    default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
+  }
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(forward(st, S.stream, true));
+  const SpinorPtr an = act_ptr(nl);
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_net_extract<P64>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
+    case P32: hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
+    default: hipLaunchKernelGGL(k_net_extract<P16>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- training
+// Mhat (FP32) on odd fields through the lattice's one-system batch
+static int mhat32(Lattice &lat, SpinorPtr out, SpinorPtr x, SpinorPtr tmp, hipStream_t s) {
+  SpinorPtr none{};
+  const LaunchCfg cfg = tuned_cfg(lat.self, P32);
+  QCU_TRY(hop(lat.self, P32, tmp, x, 0, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, s));
+  QCU_TRY(hop(lat.self, P32, out, tmp, 1, EPI_XPAY, x, RED_NONE, none, nullptr, nullptr, cfg, s));
+  return 0;
+}
+
+// out = a * Mhat^dag r = a (r - kappa^2 g5 D_oe Ae^-1 D_eo g5 Ao^-1 r)
+static int mhat32_dag(Lattice &lat, SpinorPtr out, SpinorPtr r, SpinorPtr t1, SpinorPtr t2, float a, hipStream_t s) {
+  const int vh = lat.g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  SpinorPtr none{};
+  const LaunchCfg cfg = tuned_cfg(lat.self, P32);
+  hipLaunchKernelGGL(k_g5_clover, dim3(nb), dim3(bs), 0, s, t1, r, lat.clover_inv(P32, 1), vh);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(hop(lat.self, P32, t2, t1, 0, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, s, true));
+  QCU_TRY(hop(lat.self, P32, t1, t2, 1, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr, cfg, s, false));
+  hipLaunchKernelGGL(k_g5_xpay, dim3(nb), dim3(bs), 0, s, out, r, t1, (float)(lat.kappa * lat.kappa), a, vh);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+struct NetTrainer {
+  Net &net;
+  hipStream_t s;
+  SpinorPtr odd(int k) const {
+    SpinorPtr p;
+    p.p = (char *)net.tmp.p + (size_t)k * net.lat->g.vh * 96;
+    p.norm = nullptr;
+    p.stride = net.lat->g.vh;
+    return p;
+  }
+  // loss |Mhat K b - b|^2 / |b|^2 of one odd source already in odd(0); optionally
+  // accumulates the weight gradient (gW += sample gradient) via gsum
+  int sample(double *loss, bool grad, float *gsum) {
+    Lattice &lat = *net.lat;
+    const int vh = lat.g.vh, bs = 256;
+    const unsigned nb = (unsigned)ndiv(vh, bs);
+    // |b|^2
+    double *red = (double *)net.red.p;
+    hipLaunchKernelGGL(k_sub_norm, dim3(nb), dim3(bs), 0, s, odd(5), odd(0), odd(0), 0.f, vh, red);
+    QCU_HIP(hipGetLastError());
+    std::vector<double> h(nb);
+    QCU_HIP(hipMemcpyAsync(h.data(), red, nb * 8, hipMemcpyDeviceToHost, s));
+    QCU_HIP(hipStreamSynchronize(s));
+    double b2 = 0;
+    for (double x : h) b2 += x;
+    if (!(b2 > 0)) {
+      *loss = 0;
+      return 0;
+    }
+    // y = K b / |b|
+    hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), odd(0), vh, nullptr);
+    QCU_HIP(hipGetLastError());
+    QCU_TRY(net.forward(nullptr, s, false));
+    hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, odd(1), net.act_ptr(net.nl), vh, nullptr);
+    QCU_HIP(hipGetLastError());
+    QCU_TRY(mhat32(lat, odd(2), odd(1), odd(3), s));
+    // res = Mhat K b - b, unnormalised, then scaled
+    hipLaunchKernelGGL(k_sub_norm, dim3(nb), dim3(bs), 0, s, odd(4), odd(2), odd(0), 1.f, vh, red);
+    QCU_HIP(hipGetLastError());
+    QCU_HIP(hipMemcpyAsync(h.data(), red, nb * 8, hipMemcpyDeviceToHost, s));
+    QCU_HIP(hipStreamSynchronize(s));
+    double r2 = 0;
+    for (double x : h) r2 += x;
+    *loss = r2 / b2;
+    if (!grad) return 0;
+    // dL/d(K b) on odd sites = 2 Mhat^dag res / |b|^2 ; zero on even sites
+    QCU_TRY(mhat32_dag(lat, odd(2), odd(4), odd(3), odd(5), (float)(2.0 / b2), s));
+    SpinorPtr gl;
+    gl.p = net.gbuf[1].p;
+    gl.norm = nullptr;
+    gl.stride = vh;
+    hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, gl, odd(2), vh, nullptr);
+    QCU_HIP(hipGetLastError());
+    // the backward pass ping-pongs through gbuf[0] and gbuf[1]; the last layer's
+    // output gradient lives in gbuf[1] and is consumed first
+    SpinorPtr none{};
+    QCU_TRY(net.backward(gl, s, false, none));
+    (void)gsum;
+    return 0;
+  }
+};
+
+}  // namespace qcu
+
+using namespace qcu;
+
+struct QcuNet_s {
+  Net net;
+};
+
+namespace qcu {
+int net_precond(void *handle, const Lattice *lat, Precond **out) {
+  QcuNet_s *h = (QcuNet_s *)handle;
+  QCU_CHECK(h->net.lat == lat, QCU_ERR_ARG, "the preconditioner was built for another lattice");
+  *out = &h->net;
+  return 0;
+}
+}  // namespace qcu
+
+extern "C" {
+
+int qcuNetCreate(QcuNetHandle *h, QcuLatticeHandle lat, int nlayers, const int *chan, uint64_t seed,
+                 double init_scale) {
+  clear_error();
+  QCU_CHECK(h && lat && chan, QCU_ERR_ARG, "null argument");
+  std::unique_ptr<QcuNet_s> n(new QcuNet_s());
+  QCU_TRY(n->net.init(&lat->lat, nlayers, chan, seed, init_scale));
+  *h = n.release();
+  return 0;
+}
+
+int qcuNetDestroy(QcuNetHandle h) {
+  clear_error();
+  delete h;
+  return 0;
+}
+
+long qcuNetWeightCount(QcuNetHandle h) { return h ? (long)h->net.nw : 0; }
+
+// weights as complex128 [n] on the host
+int qcuNetGetWeights(QcuNetHandle h, double *host, long n) {
+  clear_error();
+  QCU_CHECK(h && host && n == (long)h->net.nw, QCU_ERR_ARG, "weight count mismatch");
+  std::vector<float> f(2 * n);
+  QCU_HIP(hipMemcpy(f.data(), h->net.W.p, n * 8, hipMemcpyDeviceToHost));
+  for (long k = 0; k < 2 * n; ++k) host[k] = f[k];
+  return 0;
+}
+
+int qcuNetSetWeights(QcuNetHandle h, const double *host, long n) {
+  clear_error();
+  QCU_CHECK(h && host && n == (long)h->net.nw, QCU_ERR_ARG, "weight count mismatch");
+  std::vector<float> f(2 * n);
+  for (long k = 0; k < 2 * n; ++k) f[k] = (float)host[k];
+  QCU_HIP(hipMemcpy(h->net.W.p, f.data(), n * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int qcuNetGetGradient(QcuNetHandle h, double *host, long n) {
+  clear_error();
+  QCU_CHECK(h && host && n == (long)h->net.nw, QCU_ERR_ARG, "weight count mismatch");
+  std::vector<float> f(2 * n);
+  QCU_HIP(hipMemcpy(f.data(), h->net.gW.p, n * 8, hipMemcpyDeviceToHost));
+  for (long k = 0; k < 2 * n; ++k) host[k] = f[k];
+  return 0;
+}
+
+// full-lattice forward: in [1][2][Vh][12], out [1][2][Vh][12] complex128 (device)
+int qcuNetForward(QcuNetHandle h, void *out, const void *in) {
+  clear_error();
+  QCU_CHECK(h && out && in, QCU_ERR_ARG, "null argument");
+  Net &net = h->net;
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh;
+  const long n0 = (long)net.chan[0] * 2 * vh;
+  hipLaunchKernelGGL(k_full_in, dim3((unsigned)ndiv(n0, 256)), dim3(256), 0, s, net.act_ptr(0),
+                     (const double2 *)in, net.chan[0], vh);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(net.forward(nullptr, s, false));
+  const long nn = (long)net.chan[net.nl] * 2 * vh;
+  hipLaunchKernelGGL(k_full_out, dim3((unsigned)ndiv(nn, 256)), dim3(256), 0, s, (double2 *)out,
+                     net.act_ptr(net.nl), net.chan[net.nl], vh);
+  QCU_HIP(hipGetLastError());
+  QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// full-lattice backward for a given input and output gradient: fills the
+// weight gradient (read with qcuNetGetGradient) and grad_in (may be null)
+int qcuNetBackward(QcuNetHandle h, void *grad_in, const void *grad_out, const void *in) {
+  clear_error();
+  QCU_CHECK(h && grad_out && in, QCU_ERR_ARG, "null argument");
+  Net &net = h->net;
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh;
+  const long n0 = (long)net.chan[0] * 2 * vh;
+  hipLaunchKernelGGL(k_full_in, dim3((unsigned)ndiv(n0, 256)), dim3(256), 0, s, net.act_ptr(0),
+                     (const double2 *)in, net.chan[0], vh);
+  QCU_TRY(net.forward(nullptr, s, false));
+  SpinorPtr go;
+  go.p = net.gbuf[1].p;
+  go.norm = nullptr;
+  go.stride = vh;
+  const long nn = (long)net.chan[net.nl] * 2 * vh;
+  hipLaunchKernelGGL(k_full_in, dim3((unsigned)ndiv(nn, 256)), dim3(256), 0, s, go, (const double2 *)grad_out,
+                     net.chan[net.nl], vh);
+  QCU_HIP(hipGetLastError());
+  SpinorPtr gi;
+  gi.p = net.tmp.p;  // room for one channel on both parities (6 odd vectors)
+  gi.norm = nullptr;
+  gi.stride = vh;
+  QCU_TRY(net.backward(go, s, grad_in != nullptr, gi));
+  if (grad_in) {
+    hipLaunchKernelGGL(k_full_out, dim3((unsigned)ndiv(n0, 256)), dim3(256), 0, s, (double2 *)grad_in, gi,
+                       net.chan[0], vh);
+    QCU_HIP(hipGetLastError());
+  }
+  QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Trains on odd-site sources srcs[k] (external one-parity complex128 fields on
+// the device, e.g. bhat of the Schur system): `epochs` passes, Adam, mini-batches
+// of `batch` sources.  losses[e] = mean loss of epoch e (before the updates of
+// that epoch's batches are applied in turn).  seconds: wall time.
+int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int batch, double lr, double *losses,
+                double *seconds) {
+  clear_error();
+  QCU_CHECK(h && srcs && n > 0 && epochs > 0 && batch > 0, QCU_ERR_ARG, "bad training arguments");
+  Net &net = h->net;
+  Lattice &lat = *net.lat;
+  hipStream_t s = lat.stream;
+  const int vh = lat.g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  NetTrainer tr{net, s};
+  DevBuf gsum;
+  QCU_TRY(gsum.alloc(net.nw * 8));
+  auto t0 = std::chrono::steady_clock::now();
+  const int nfl = (int)(2 * net.nw);
+  for (int e = 0; e < epochs; ++e) {
+    double esum = 0;
+    for (int b0 = 0; b0 < n; b0 += batch) {
+      const int b1 = std::min(n, b0 + batch);
+      QCU_HIP(hipMemsetAsync(gsum.p, 0, net.nw * 8, s));
+      for (int k = b0; k < b1; ++k) {
+        QCU_CHECK(srcs[k], QCU_ERR_ARG, "null source");
+        hipLaunchKernelGGL(k_load_odd, dim3(nb), dim3(bs), 0, s, tr.odd(0), (const double2 *)srcs[k], vh);
+        QCU_HIP(hipGetLastError());
+        double loss = 0;
+        QCU_TRY(tr.sample(&loss, true, nullptr));
+        esum += loss;
+        // gsum += gW
+        hipLaunchKernelGGL(k_add_f32, dim3((unsigned)ndiv(nfl, 256)), dim3(256), 0, s, (float *)gsum.p,
+                           (const float *)net.gW.p, nfl);
+        QCU_HIP(hipGetLastError());
+      }
+      net.adam_t += 1;
+      const float b1f = 0.9f, b2f = 0.999f;
+      const float c1 = 1.f - powf(b1f, (float)net.adam_t), c2 = 1.f - powf(b2f, (float)net.adam_t);
+      hipLaunchKernelGGL(k_adam, dim3((unsigned)ndiv(nfl, 256)), dim3(256), 0, s, (float *)net.W.p, (float *)net.m.p,
+                         (float *)net.v.p, (const float *)gsum.p, nfl, (float)lr, b1f, b2f, c1, c2, 1e-8f,
+                         1.f / (float)(b1 - b0));
+      QCU_HIP(hipGetLastError());
+    }
+    if (losses) losses[e] = esum / n;
+  }
+  QCU_HIP(hipStreamSynchronize(s));
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
+  clear_error();
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_TRY(check_dev_ptr(out, "out"));
+  QCU_TRY(check_dev_ptr(in, "in"));
+  Net &net = h->net;
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  NetTrainer tr{net, s};
+  hipLaunchKernelGGL(k_load_odd, dim3(nb), dim3(bs), 0, s, tr.odd(0), (const double2 *)in, vh);
+  QCU_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(net.forward(nullptr, s, true));
+  SpinorPtr o1 = net.act_ptr(net.nl);
+  o1.p = (char *)o1.p + (size_t)vh * 96;  // channel 0, parity 1
+  hipLaunchKernelGGL(k_store_odd, dim3(nb), dim3(bs), 0, s, (double2 *)out, o1, vh);
+  QCU_HIP(hipGetLastError());
+  QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss) {
+  clear_error();
+  QCU_CHECK(h && loss, QCU_ERR_ARG, "null argument");
+  QCU_TRY(check_dev_ptr(src, "src"));
+  Net &net = h->net;
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh, bs = 256;
+  NetTrainer tr{net, s};
+  hipLaunchKernelGGL(k_load_odd, dim3((unsigned)ndiv(vh, bs)), dim3(bs), 0, s, tr.odd(0), (const double2 *)src, vh);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(tr.sample(loss, true, nullptr));
+  QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// mean loss over sources without training
+int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss) {
+  clear_error();
+  QCU_CHECK(h && srcs && n > 0 && loss, QCU_ERR_ARG, "bad arguments");
+  Net &net = h->net;
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh, bs = 256;
+  NetTrainer tr{net, s};
+  double sum = 0;
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_load_odd, dim3((unsigned)ndiv(vh, bs)), dim3(bs), 0, s, tr.odd(0), (const double2 *)srcs[k],
+                       vh);
+    QCU_HIP(hipGetLastError());
+    double l = 0;
+    QCU_TRY(tr.sample(&l, false, nullptr));
+    sum += l;
+  }
+  *loss = sum / n;
+  return 0;
+}
+
+}  // extern "C"

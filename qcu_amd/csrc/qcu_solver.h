@@ -55,6 +55,12 @@ struct SolveStats {
 LaunchCfg tuned_cfg(const BatchOp &op, int prec);
 std::string tune_key(const BatchOp &op, int prec);
 
+// 0 when p is a HIP device (or managed) pointer; else records the error
+int check_dev_ptr(const void *p, const char *what);
+
+// the preconditioner behind a QcuNetHandle; it must belong to lattice lat
+int net_precond(void *handle, const Lattice *lat, Precond **out);
+
 // b[k], x[k]: full-lattice external fields of system k (device pointers)
 int full_solve(Solver &S, void *const *x_ext, const void *const *b_ext, const QcuSolveParam &prm,
                QcuSolveInfo *info);

@@ -218,3 +218,104 @@ def tune_save(path):
 
 def tune_load(path):
     _lib.check(_lib.load().qcuTuneLoad(str(path).encode()))
+
+
+class LearnedPreconditioner:
+    """Gauge-covariant linear network K on one lattice (see qcuNetCreate).
+
+    channels: (1, 8, 8, 8, 1) gives 4 layers with 8 hidden channels.  Weights
+    are complex, in the order layer, term (2 mu forward hop, 2 mu + 1 backward
+    hop, 8 site), output channel, input channel, 4x4 spin matrix.  Pass the
+    object as ``precond=`` to :meth:`Lattice.solve`.
+    """
+
+    def __init__(self, lattice: Lattice, channels=(1, 8, 8, 8, 1), seed=1, init_scale=0.05):
+        self._lib = _lib.load()
+        self.lattice = lattice
+        self.channels = tuple(int(c) for c in channels)
+        ch = (C.c_int * len(self.channels))(*self.channels)
+        h = C.c_void_p()
+        _lib.check(self._lib.qcuNetCreate(C.byref(h), lattice.h, len(self.channels) - 1, ch,
+                                          int(seed), float(init_scale)))
+        self.handle = h
+        self.nweights = int(self._lib.qcuNetWeightCount(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.qcuNetDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _host_buf(self):
+        import numpy as np
+        return np.zeros(self.nweights, dtype=np.complex128)
+
+    def get_weights(self):
+        w = self._host_buf()
+        _lib.check(self._lib.qcuNetGetWeights(self.handle, w.ctypes.data_as(C.POINTER(C.c_double)),
+                                              self.nweights))
+        return w
+
+    def set_weights(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128).reshape(-1))
+        if w.size != self.nweights:
+            raise ValueError(f"expected {self.nweights} weights, got {w.size}")
+        _lib.check(self._lib.qcuNetSetWeights(self.handle, w.ctypes.data_as(C.POINTER(C.c_double)),
+                                              self.nweights))
+
+    def gradient(self):
+        g = self._host_buf()
+        _lib.check(self._lib.qcuNetGetGradient(self.handle, g.ctypes.data_as(C.POINTER(C.c_double)),
+                                               self.nweights))
+        return g
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        """K x on a full-lattice field"""
+        out = torch.empty_like(x)
+        _lib.check(self._lib.qcuNetForward(self.handle, _ptr(out), _ptr(x)))
+        return out
+
+    def backward(self, x: torch.Tensor, grad_out: torch.Tensor, want_input_grad=True):
+        """(weight gradient as complex128 numpy, input gradient or None) for a
+        real loss with dL/dRe(Kx) + i dL/dIm(Kx) = grad_out"""
+        gi = torch.empty_like(x) if want_input_grad else None
+        _lib.check(self._lib.qcuNetBackward(self.handle, None if gi is None else _ptr(gi),
+                                            _ptr(grad_out), _ptr(x)))
+        return self.gradient(), gi
+
+    def apply_odd(self, x_odd: torch.Tensor) -> torch.Tensor:
+        """K on an odd-site field (even sites taken as zero), as in the solver"""
+        out = torch.empty_like(x_odd)
+        _lib.check(self._lib.qcuNetApplyOdd(self.handle, _ptr(out), _ptr(x_odd)))
+        return out
+
+    def train(self, sources, epochs=10, batch=16, lr=1e-3):
+        """Adam on |Mhat K b - b|^2 / |b|^2 over odd-site sources; returns
+        (per-epoch mean losses, seconds)"""
+        import numpy as np
+        n = len(sources)
+        sp = (C.c_void_p * n)(*[_ptr(s).value for s in sources])
+        losses = np.zeros(epochs)
+        secs = C.c_double()
+        _lib.check(self._lib.qcuNetTrain(self.handle, sp, n, int(epochs), int(batch), float(lr),
+                                         losses.ctypes.data_as(C.POINTER(C.c_double)), C.byref(secs)))
+        return losses, secs.value
+
+    def loss_grad(self, source):
+        """(loss, weight gradient) of |Mhat K b - b|^2 / |b|^2 for one odd source"""
+        out = C.c_double()
+        _lib.check(self._lib.qcuNetLossGrad(self.handle, _ptr(source), C.byref(out)))
+        return out.value, self.gradient()
+
+    def loss(self, sources) -> float:
+        n = len(sources)
+        sp = (C.c_void_p * n)(*[_ptr(s).value for s in sources])
+        out = C.c_double()
+        _lib.check(self._lib.qcuNetLoss(self.handle, sp, n, C.byref(out)))
+        return out.value

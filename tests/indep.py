@@ -173,3 +173,24 @@ def gauge_from_lexico(Ul, L):
     """[4][Lt][Lz][Ly][Lx][3][3] -> external [4][2][Vh][3][3]"""
     return np.ascontiguousarray(np.stack([to_cb(Ul[mu], L).reshape(2, -1, 3, 3)
                                           for mu in range(4)]))
+
+
+def net_lexico(inp, Ws, Ul, L, t_boundary=1):
+    """Gauge-covariant linear network on lexicographic channel fields.
+
+    inp: [cin][Lt][Lz][Ly][Lx][4][3]; Ws: per layer [9][cout][cin][4][4] with
+    term 2 mu = W U_mu(x) psi(x+mu), 2 mu + 1 = W U_mu(x-mu)^+ psi(x-mu), 8 = W psi(x).
+    """
+    a = inp
+    for W in Ws:
+        terms = []
+        for mu in range(4):
+            ax = AX[mu]
+            f = np.einsum("tzyxab,ctzyxsb->ctzyxsa", Ul[mu], np.roll(a, -1, axis=ax + 1))
+            terms.append(_phase(L, mu, True, t_boundary)[None, ..., None, None] * f)
+            Ub = np.roll(Ul[mu], 1, axis=ax)
+            b = np.einsum("tzyxba,ctzyxsb->ctzyxsa", Ub.conj(), np.roll(a, 1, axis=ax + 1))
+            terms.append(_phase(L, mu, False, t_boundary)[None, ..., None, None] * b)
+        terms.append(a)
+        a = sum(np.einsum("oirs,itzyxsc->otzyxrc", W[t], terms[t]) for t in range(9))
+    return a

@@ -1,0 +1,147 @@
+"""GPU parity of the learned preconditioner's HIP kernels against the oracle.
+
+Forward, input gradient and weight gradient are compared with the oracle's
+network (itself pinned in test_oracle_net.py) on the same FP32-rounded
+weights; the training loss gradient is compared with the oracle chain
+K -> Mhat -> |.|^2 whose Mhat^+ is pinned here by the adjoint identity.
+The last test trains on the GPU and shows the iteration gain of the
+preconditioned solve, every solve checked by the oracle's FP64 true residual.
+FP32 tolerances are stated in each test.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import dev, host, log, rel
+
+pytestmark = pytest.mark.gpu
+
+G5 = np.array([1, 1, -1, -1], float)
+
+
+def _w32(w):
+    return w.astype(np.complex64).astype(np.complex128)
+
+
+def _setup(orc, L, chan, tb, seed, csw=0.0, kappa=0.124, eps=0.4):
+    from qcu_amd.lattice import Lattice, LearnedPreconditioner
+    U = orc.gauge_random(L, eps, seed)
+    lat = Lattice(L, kappa, csw, tb)
+    lat.load_gauge(dev(U))
+    net = LearnedPreconditioner(lat, chan, seed=seed)
+    W = _w32(orc.gauss((net.nweights,), seed + 1) * 0.3)
+    net.set_weights(W)
+    return U, lat, net, W
+
+
+@pytest.mark.parametrize("L,chan,tb", [((8, 4, 4, 8), (1, 8, 8, 8, 1), -1), ((4, 6, 4, 4), (1, 2, 3, 1), 1),
+                                       ((6, 4, 6, 2), (1, 1), -1)])
+def test_net_forward_and_backward_vs_oracle(orc, L, chan, tb):
+    U, lat, net, W = _setup(orc, L, chan, tb, 100)
+    assert net.nweights == orc.net_weight_count(chan)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 101)
+    g = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 102)
+    ref, acts = orc.net_forward(x, W, chan, U, L, tb)
+    out = host(net.forward(dev(x)))
+    e_fwd = rel(out.ravel(), ref)
+    gin_ref, gW_ref = orc.net_backward(g, W, chan, U, L, acts, tb)
+    gW, gin = net.backward(dev(x), dev(g))
+    e_gin = rel(host(gin).ravel(), gin_ref)
+    e_gw = rel(gW, gW_ref)
+    log(f"L={L} chan={chan} tb={tb}: forward {e_fwd:.1e}, input grad {e_gin:.1e}, weight grad {e_gw:.1e}")
+    # FP32 activations and weights, sums of at most 9*8 terms per layer
+    assert e_fwd < 2e-5 and e_gin < 2e-5
+    # weight gradient: FP32 site products, blockwise FP32 sums, FP64 across blocks
+    assert e_gw < 1e-4
+
+
+def test_net_apply_odd_is_odd_part_of_forward(orc):
+    L, chan = (8, 8, 4, 4), (1, 4, 4, 1)
+    U, lat, net, W = _setup(orc, L, chan, -1, 110)
+    xo = orc.gauss((orc.vol(L) // 2, 4, 3), 111)
+    full = np.zeros((1, 2) + xo.shape, complex)
+    full[0, 1] = xo
+    ref, _ = orc.net_forward(full, W, chan, U, L, -1)
+    got = host(net.apply_odd(dev(xo)))
+    e = rel(got.ravel(), ref.reshape(2, -1)[1])
+    log(f"apply_odd vs oracle {e:.1e}")
+    assert e < 2e-5
+
+
+def _schur_ops(orc, U, L, kappa, csw, tb):
+    """Mhat and Mhat^+ on odd vectors from oracle pieces"""
+    coeff = kappa * csw
+    if csw:
+        Ao_inv = orc.clover_invert(orc.clover_build(U, L, 1, coeff))
+    else:
+        Ao_inv = None
+
+    def mhat(y):
+        t = orc.clover_dslash(y, U, L, 0, coeff, tb)
+        return y - kappa ** 2 * orc.clover_dslash(t, U, L, 1, coeff, tb)
+
+    def mhat_dag(r):
+        z = r if Ao_inv is None else orc.clover_apply(Ao_inv, r)
+        z = z * G5[None, :, None]
+        t = orc.clover_dslash(z, U, L, 0, coeff, tb)
+        t = orc.hop(t, U, L, 1, tb)
+        return r - kappa ** 2 * t * G5[None, :, None]
+
+    return mhat, mhat_dag
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_training_gradient_vs_oracle(orc, csw):
+    L, chan, tb, kappa = (8, 4, 4, 8), (1, 4, 4, 1), -1, 0.124
+    U, lat, net, W = _setup(orc, L, chan, tb, 120, csw=csw, kappa=kappa)
+    net.set_weights(W * 0.2)  # a smaller net keeps Mhat K b near b
+    W = net.get_weights()
+    mhat, mhat_dag = _schur_ops(orc, U, L, kappa, csw, tb)
+    vh = orc.vol(L) // 2
+    u = orc.gauss((vh, 4, 3), 121)
+    v = orc.gauss((vh, 4, 3), 122)
+    adj = abs(np.vdot(u, mhat(v)) - np.vdot(mhat_dag(u), v)) / abs(np.vdot(u, mhat(v)))
+    assert adj < 1e-13  # pins the Mhat^+ used below
+    b = _w32(orc.gauss((vh, 4, 3), 123))
+    full = np.zeros((1, 2, vh, 4, 3), complex)
+    full[0, 1] = b
+    y, acts = orc.net_forward(full, W, chan, U, L, tb)
+    y_odd = y.reshape(2, vh, 4, 3)[1]
+    res = mhat(y_odd) - b
+    b2 = np.vdot(b, b).real
+    loss_ref = np.vdot(res, res).real / b2
+    gy = np.zeros((1, 2, vh, 4, 3), complex)
+    gy[0, 1] = 2 * mhat_dag(res) / b2
+    _, gW_ref = orc.net_backward(gy, W, chan, U, L, acts, tb)
+    loss, gW = net.loss_grad(dev(b))
+    e_l, e_g = abs(loss - loss_ref) / loss_ref, rel(gW, gW_ref)
+    log(f"csw={csw}: loss {loss:.6f} (oracle {loss_ref:.6f}, rel {e_l:.1e}), gradient rel {e_g:.1e}")
+    # FP32 operator and network; the loss is a difference of O(1) terms
+    assert e_l < 1e-4 and e_g < 1e-3
+
+
+def test_trained_preconditioner_reduces_iterations(orc):
+    """Adam on the GPU lowers the loss; the preconditioned solve needs fewer
+    iterations than the plain one and still meets |b - M x|/|b| <= 1e-10"""
+    from qcu_amd.lattice import FP32, Lattice, LearnedPreconditioner
+    L, kappa, csw, tb = (8, 8, 8, 16), 0.1245, 1.0, -1
+    U = orc.gauge_random(L, 0.25, 130)
+    lat = Lattice(L, kappa, csw, tb)
+    lat.load_gauge(dev(U))
+    vh = orc.vol(L) // 2
+    srcs = [dev(orc.gauss((vh, 4, 3), 1000 + k)) for k in range(32)]
+    net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3)
+    l0 = net.loss(srcs)
+    losses, secs = net.train(srcs, epochs=25, batch=4, lr=2e-3)
+    l1 = net.loss(srcs)
+    log(f"loss {l0:.4f} -> {l1:.4f} in {secs:.1f} s; per epoch {np.round(losses, 4).tolist()}")
+    assert l1 < 0.8 * l0
+    b = dev(orc.gauss((2, vh, 4, 3), 140))
+    x0, r0 = lat.solve(b, tol=1e-10, prec=FP32)
+    x1, r1 = lat.solve(b, tol=1e-10, prec=FP32, precond=net)
+    t0 = orc.true_residual(host(x0), host(b), U, L, kappa, csw, tb)
+    t1 = orc.true_residual(host(x1), host(b), U, L, kappa, csw, tb)
+    log(f"plain {r0.iters} iters ({r0.seconds * 1e3:.1f} ms, true res {t0:.1e}); "
+        f"learned {r1.iters} iters ({r1.seconds * 1e3:.1f} ms, true res {t1:.1e}, fallback {r1.fallback})")
+    assert r0.converged and r1.converged and t0 <= 1e-10 and t1 <= 1e-10
+    assert not r1.fallback
+    assert r1.iters < r0.iters
