@@ -1,12 +1,14 @@
 """Headline benchmark: time-to-solution at 1e-10 true residual on a 32^3x64
-clover lattice (gauge eps 0.2, csw 1.0, kappa 0.1248, antiperiodic t), fast
-path (learned preconditioner when trained, mixed FP16-with-scale inner
-BiCGSTAB with FP64 reliable updates) against the plain FP64 even-odd BiCGSTAB
-on the same GPU.
+clover lattice (gauge eps 0.2, csw 1.0, kappa 0.1248, antiperiodic t) of the
+learned-preconditioned mixed-precision solve (4-layer gauge-covariant network
+trained here on the GPU, FP16-with-scale inner BiCGSTAB, FP64 reliable
+updates) against the plain FP64 even-odd BiCGSTAB on the same GPU.
 
-A step is one complete solve of the fast path on one Gaussian source that is
-already resident in HBM (external PyQuda layout).  The plain FP64 solve of the
-same source is timed alongside; value = t_plain / t_fast.
+A step is one complete learned+mixed solve of one Gaussian source already
+resident in HBM (external PyQuda layout).  The plain FP64 solve and the mixed
+solve without the network are timed alongside (median of `steps` solves);
+value = t_plain / t_learned_mixed.  Training time is reported, not timed in
+the step.
 
 Prints one JSON line (rank 0, N = 1).  --gpus N > 1 exits: this build runs on
 one GPU.
@@ -25,6 +27,7 @@ L = (32, 32, 32, 64)
 EPS, CSW, KAPPA, TB = 0.2, 1.0, 0.1248, -1
 TOL = 1e-10
 PEAK_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
 # algorithmic bytes per output site of one hop launch (8 neighbour spinors,
 # 8 compressed links, 1 output) and of the packed clover inverse, per precision
 HOP_BYTES = {0: 8 * 192 + 8 * 96 + 192, 1: 8 * 96 + 8 * 48 + 96, 2: 8 * 52 + 8 * 24 + 52}
@@ -67,7 +70,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner", default="fp16", choices=["fp32", "fp16"])
-    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--delta", type=float, default=0.01, help="reliable-update residual drop")
+    ap.add_argument("--train-sources", type=int, default=256)
+    ap.add_argument("--train-epochs", type=int, default=1)
+    ap.add_argument("--train-batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -75,54 +82,66 @@ def main():
         sys.exit(2)
 
     import torch
-    from qcu_amd.lattice import FP16, FP32, FP64, Lattice, gaussian, random_gauge
+    from qcu_amd.lattice import FP16, FP32, FP64, Lattice, LearnedPreconditioner, gaussian, random_gauge
 
     inner = FP16 if args.inner == "fp16" else FP32
     U = random_gauge(L, EPS, 20241)
     lat = Lattice(L, KAPPA, CSW, TB)
     lat.load_gauge(U)
+    vh = lat.vh
     b = gaussian((2,) + (L[3], L[2], L[1], L[0] // 2, 4, 3), 777)
     torch.cuda.synchronize()
     log("gauge + clover resident; tuning hop launch shapes")
     for p in (FP64, inner):
         log(f"  prec {p}: {lat.tune(p)}")
 
-    def fast():
-        return lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta)
+    def timed(fn, n):
+        ts, res = [], None
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, res = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            assert res.converged and res.true_res <= TOL, f"solve did not reach {TOL}: {res}"
+        return sorted(ts)[len(ts) // 2], res
 
-    def plain():
-        return lat.solve(b, tol=TOL, prec=FP64)
+    # plain FP64 BiCGSTAB and the mixed-precision solve without the network
+    plain = lambda: lat.solve(b, tol=TOL, prec=FP64)
+    mixed = lambda: lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta)
+    plain()
+    mixed()
+    t_plain, plain_res = timed(plain, max(1, args.steps))
+    log(f"plain FP64: {t_plain * 1e3:.1f} ms {plain_res}")
+    t_mixed, mixed_res = timed(mixed, max(1, args.steps))
+    log(f"mixed {args.inner}: {t_mixed * 1e3:.1f} ms {mixed_res}")
 
-    for _ in range(args.warmup):
-        fast()
-    x, r = plain()
-    log(f"plain warmup: {r}")
+    # the learned preconditioner, trained on the GPU on Gaussian odd-site sources
+    net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3)
+    srcs = [gaussian((vh, 4, 3), 10000 + k) for k in range(args.train_sources)]
     torch.cuda.synchronize()
-    t_plain = []
-    plain_res = None
-    for _ in range(max(1, args.steps)):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x, plain_res = plain()
-        torch.cuda.synchronize()
-        t_plain.append(time.perf_counter() - t0)
-    log(f"plain: {plain_res}")
+    losses, train_s = net.train(srcs, epochs=args.train_epochs, batch=args.train_batch, lr=args.lr)
+    del srcs
+    log(f"training: {train_s:.1f} s, epoch losses {losses.tolist()}")
+    learned = lambda: lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta, precond=net)
+    for _ in range(args.warmup):
+        learned()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fast_res = []
+    lres = []
     for _ in range(args.steps):
-        x, r = fast()
-        fast_res.append(r)
+        _, r = learned()
+        lres.append(r)
     torch.cuda.synchronize()
-    t_fast = (time.perf_counter() - t0) / args.steps
-    log(f"fast: {fast_res[-1]}")
-    t_plain_med = sorted(t_plain)[len(t_plain) // 2]
-    assert all(r.converged and r.true_res <= TOL for r in fast_res), "fast solve did not reach 1e-10"
-    assert plain_res.converged and plain_res.true_res <= TOL, "plain solve did not reach 1e-10"
+    t_learned = (time.perf_counter() - t0) / args.steps
+    assert all(r.converged and r.true_res <= TOL for r in lres), "learned solve did not reach 1e-10"
+    log(f"learned+mixed: {t_learned * 1e3:.1f} ms {lres[-1]}")
 
-    # roofline of the dominant kernel: the inner-precision clover hop with the
-    # xpay epilogue, timed with HIP events on the library's stream
-    vh = lat.vh
+    # roofline of the dominant kernel of the timed solve: the network's 8 -> 8
+    # channel layer (FP32 FMA, compute-bound), timed with HIP events
+    ms_layer, fl_layer = net.bench_layer(1, iters=10)
+    tflops = fl_layer / (ms_layer * 1e-3) / 1e12
+    # the inner-precision clover hop with the xpay epilogue (HBM-bound)
     ms_hop = lat.bench_hop(prec=inner, clover=True, xpay=True, iters=100)
     alg = (HOP_BYTES[inner] + CLOVER_BYTES[inner] + XPAY_BYTES[inner]) * vh
     achieved = alg / (ms_hop * 1e-3) / 1e9
@@ -141,33 +160,43 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": round(t_plain_med / t_fast, 3),
-        "unit": "x (plain FP64 time / fast time)",
+        "value": round(t_plain / t_learned, 4),
+        "unit": "x (plain FP64 time-to-solution / learned+mixed time-to-solution)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_fast * 1e3, 3),
+        "ms_per_step": round(t_learned * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (outer) / " + args.inner + " (inner)",
-        "data": "synthetic: U = exp(i 0.2 H) seeded, Gaussian source",
+        "dtype": "f64 (outer) / " + args.inner + " (inner) / f32 (network)",
+        "data": "synthetic: U = exp(i 0.2 H) seeded, Gaussian source; network trained here on "
+                f"{args.train_sources} Gaussian sources",
         "config": {"workload": "32^3x64 clover, eps 0.2, csw 1.0, kappa 0.1248, antiperiodic t, "
                                "tol 1e-10 true residual",
-                   "fast_path": f"mixed {args.inner} inner BiCGSTAB, FP64 reliable updates (delta "
-                                f"{args.delta}); learned preconditioner: not applied",
+                   "fast_path": f"learned preconditioner (4 layers, 8 channels, FP32) + mixed "
+                                f"{args.inner} inner BiCGSTAB, FP64 reliable updates (delta {args.delta})",
                    "plain_path": "FP64 even-odd BiCGSTAB"},
-        "tts_fast_s": round(t_fast, 4),
-        "tts_plain_s": round(t_plain_med, 4),
-        "iters_fast": fast_res[-1].iters,
-        "reliable_updates_fast": fast_res[-1].reliable_updates,
+        "tts_learned_mixed_s": round(t_learned, 4),
+        "tts_mixed_s": round(t_mixed, 4),
+        "tts_plain_s": round(t_plain, 4),
+        "speedup_mixed_only": round(t_plain / t_mixed, 3),
+        "iters_learned_mixed": lres[-1].iters,
+        "iters_mixed": mixed_res.iters,
         "iters_plain": plain_res.iters,
-        "true_res_fast": fast_res[-1].true_res,
-        "true_res_plain": plain_res.true_res,
-        "roofline": {"kernel": f"hop {args.inner} clover xpay", "bound": "hbm",
-                     "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,
-                     "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
+        "true_res": {"learned_mixed": lres[-1].true_res, "mixed": mixed_res.true_res,
+                     "plain": plain_res.true_res},
+        "training": {"seconds": round(train_s, 2), "sources": args.train_sources,
+                     "epochs": args.train_epochs, "batch": args.train_batch,
+                     "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
+        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer<4>)", "bound": "mfma",
+                     "compute": "FP32 vector FMA", "achieved": round(tflops, 2), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_TFLOPS, 3), "traffic": None,
+                     "ms_per_launch": round(ms_layer, 4), "alg_flops_per_launch": fl_layer},
+        "hop_roofline": {"kernel": f"hop {args.inner} clover xpay", "bound": "hbm",
+                         "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,
+                         "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
         "hop_fp64_clover_xpay": {"ms_per_launch": round(ms64, 5),
                                  "achieved_GBps": round(alg64 / (ms64 * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,

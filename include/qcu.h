@@ -181,6 +181,9 @@ int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int 
 int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss);
 /* mean loss over the sources, no update */
 int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss);
+/* ms per launch of one layer's forward kernel (both parities) and its
+ * algorithmic flops per launch, timed with HIP events on the lattice stream */
+int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *flops);
 /* out = K in on one-parity (odd) external fields, as applied inside the solver */
 int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "qcuNetLoss": (C.c_int, [_VP, C.POINTER(_VP), C.c_int, C.POINTER(C.c_double)]),
     "qcuNetApplyOdd": (C.c_int, [_VP, _VP, _VP]),
     "qcuNetLossGrad": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
+    "qcuNetBenchLayer": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 
 

@@ -124,8 +124,9 @@ QCU_D void transport4(const cf (&u)[9], cf (&v)[12], int fwd, float ph) {
   }
 }
 
-// grid: x sites, y output parity, z output-channel group of CO channels
-template <int CO>
+// grid: x sites, y output parity, z output-channel group of CO channels;
+// CI: input channel count known at compile time (0: a.cin at run time)
+template <int CO, int CI>
 __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
   for (int o = 0; o < CO; ++o)
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
-  const int cin = a.cin, cout = a.cout;
+  const int cin = CI > 0 ? CI : a.cin, cout = a.cout;
 #pragma unroll 1
   for (int t = 0; t < 9; ++t) {
     const TermGeo tg = term_geo(g, i, p, t);
@@ -542,12 +543,14 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   const int bs = 128;
   const int co = cout >= 4 ? 4 : cout;
   dim3 grid((unsigned)ndiv(lat->g.vh, bs), nparity, (unsigned)ndiv(cout, co));
-  switch (co) {
-    case 1: hipLaunchKernelGGL(k_net_layer<1>, grid, dim3(bs), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_net_layer<2>, grid, dim3(bs), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_net_layer<3>, grid, dim3(bs), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_net_layer<4>, grid, dim3(bs), 0, s, a); break;
-  }
+  const int ci = cin == 1 ? 1 : cin == 8 ? 8 : 0;
+#define QCU_NET_CASE(CO_, CI_) \
+  if (co == CO_ && ci == CI_) hipLaunchKernelGGL((k_net_layer<CO_, CI_>), grid, dim3(bs), 0, s, a);
+  QCU_NET_CASE(1, 0) QCU_NET_CASE(1, 1) QCU_NET_CASE(1, 8)
+  QCU_NET_CASE(2, 0) QCU_NET_CASE(2, 1) QCU_NET_CASE(2, 8)
+  QCU_NET_CASE(3, 0) QCU_NET_CASE(3, 1) QCU_NET_CASE(3, 8)
+  QCU_NET_CASE(4, 0) QCU_NET_CASE(4, 1) QCU_NET_CASE(4, 8)
+#undef QCU_NET_CASE
   QCU_HIP(hipGetLastError());
   return 0;
 }
@@ -923,6 +926,43 @@ int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss) {
   QCU_HIP(hipGetLastError());
   QCU_TRY(tr.sample(loss, true, nullptr));
   QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// average ms per launch of layer `layer`'s forward kernel over both parities
+// (on the current activations), timed with HIP events on the lattice stream;
+// flops: algorithmic flops of one launch
+int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *flops) {
+  clear_error();
+  QCU_CHECK(h && ms && iters > 0, QCU_ERR_ARG, "bad arguments");
+  Net &net = h->net;
+  QCU_CHECK(layer >= 0 && layer < net.nl, QCU_ERR_ARG, "layer out of range");
+  hipStream_t s = net.lat->stream;
+  const int ci = net.chan[layer], co = net.chan[layer + 1];
+  auto launch = [&]() {
+    return net.layer(net.act_ptr(layer), net.act_ptr(layer + 1), (const float2 *)net.W.p + net.woff[layer], ci, co, 0,
+                     2, nullptr, s);
+  };
+  QCU_TRY(launch());
+  hipEvent_t e0, e1;
+  QCU_HIP(hipEventCreate(&e0));
+  QCU_HIP(hipEventCreate(&e1));
+  QCU_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < iters; ++k) QCU_TRY(launch());
+  QCU_HIP(hipEventRecord(e1, s));
+  QCU_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  QCU_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / iters;
+  if (flops) {
+    // per output site: 9 terms x cout x cin spin-matrix products on 3 colours
+    // (16 complex multiply-adds each) + 8 hops x cin transports of 4 spins
+    // (9 complex multiply-adds each); 8 real flops per complex multiply-add
+    const double per_site = 8.0 * (9.0 * co * ci * 16 * 3 + 8.0 * ci * 4 * 9);
+    *flops = per_site * 2.0 * net.lat->g.vh;
+  }
   return 0;
 }
 

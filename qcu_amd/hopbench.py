@@ -2,11 +2,13 @@
 
     python -m qcu_amd.hopbench --prec 2 --clover 1 --xpay 1 --iters 50
     python -m qcu_amd.hopbench --copy-mib 1024 --iters 50   # plain copy of known size
+    python -m qcu_amd.hopbench --net-layer 1 --iters 5       # learned-network layer 8 -> 8
 
 Runs the resident hop kernel on a 32^3x64 clover lattice (eps 0.2, csw 1.0,
 kappa 0.1248) and prints the HIP-event time per launch and the algorithmic
 GB/s.  The copy mode launches a device-to-device copy of a known byte count,
-used to calibrate the FETCH_SIZE / WRITE_SIZE counters.
+used to calibrate the FETCH_SIZE / WRITE_SIZE counters.  The net-layer mode
+times one forward layer (both parities) of the 1-8-8-8-1 channel network.
 """
 import argparse
 import json
@@ -26,6 +28,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--remap", type=int, default=-1)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--net-layer", type=int, default=-1)
     args = ap.parse_args()
 
     import torch
@@ -53,6 +56,13 @@ def main():
     lat.load_gauge(U)
     del U
     torch.cuda.empty_cache()
+    if args.net_layer >= 0:
+        from .lattice import LearnedPreconditioner
+        net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3)
+        ms, fl = net.bench_layer(args.net_layer, iters=args.iters)
+        print(json.dumps({"L": L, "net_layer": args.net_layer, "ms_per_launch": ms, "alg_flops": fl,
+                          "TFLOPs": fl / ms / 1e9}))
+        return
     ms = lat.bench_hop(prec=args.prec, clover=bool(args.clover), xpay=bool(args.xpay), iters=args.iters,
                        block=args.block, remap=args.remap)
     alg = (HOP[args.prec] + (CLOVER[args.prec] if args.clover else 0) +

@@ -313,6 +313,12 @@ class LearnedPreconditioner:
         _lib.check(self._lib.qcuNetLossGrad(self.handle, _ptr(source), C.byref(out)))
         return out.value, self.gradient()
 
+    def bench_layer(self, layer, iters=10):
+        """(ms per launch, algorithmic flops per launch) of one layer's forward kernel"""
+        ms, fl = C.c_double(), C.c_double()
+        _lib.check(self._lib.qcuNetBenchLayer(self.handle, int(layer), int(iters), C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
     def loss(self, sources) -> float:
         n = len(sources)
         sp = (C.c_void_p * n)(*[_ptr(s).value for s in sources])
