@@ -18,10 +18,10 @@ _lib = None
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the Makefile beside this file."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
-                       check=True, stdout=subprocess.DEVNULL)
+    """Compile liboracle.so with the Makefile beside this file (make rebuilds
+    it when the sources are newer; force rebuilds unconditionally)."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE, "liboracle.so"],
+                   check=True, stdout=subprocess.DEVNULL)
     return _LIB_PATH
 
 
@@ -69,6 +69,9 @@ def _declare(L):
     L.or_net_forward.argtypes = [_P, _P, _P, C.c_int, C.POINTER(C.c_int), _P, _I4, C.c_int, _P]
     L.or_net_backward.argtypes = [C.c_void_p, _P, _P, _P, C.c_int, C.POINTER(C.c_int), _P, _I4,
                                   C.c_int, _P]
+    L.or_dot_out.argtypes = [_P, _P, _P, C.c_long]
+    L.or_norm2.argtypes = [_P, C.c_long]
+    L.or_norm2.restype = C.c_double
     L.or_num_threads.restype = C.c_int
     L.or_set_num_threads.argtypes = [C.c_int]
 
@@ -166,6 +169,20 @@ def bicgstab(b, U, L, kappa, csw=0.0, t_boundary=1, tol=1e-10, max_iter=10000, m
 def true_residual(x, b, U, L, kappa, csw=0.0, t_boundary=1):
     return lib().or_true_residual(np.ascontiguousarray(x), np.ascontiguousarray(b), U, _l4(L),
                                   float(kappa), float(csw), int(t_boundary))
+
+
+def dot(a, b):
+    """<a, b> = sum conj(a) b"""
+    a = np.ascontiguousarray(a, np.complex128).reshape(-1)
+    b = np.ascontiguousarray(b, np.complex128).reshape(-1)
+    out = np.zeros(1, np.complex128)
+    lib().or_dot_out(out, a, b, a.size)
+    return complex(out[0])
+
+
+def norm2(a):
+    a = np.ascontiguousarray(a, np.complex128).reshape(-1)
+    return float(lib().or_norm2(a, a.size))
 
 
 def net_weight_count(chan):

@@ -33,9 +33,12 @@
  *                   tests draw QUDA's gaussGauge and cupy randn).  Pinned by
  *                   unitarity, det U = 1, and Gaussian moments.
  *   or_net_forward / or_net_backward  no reference counterpart (new learned
- *                   preconditioner).  Pinned by linearity, gauge covariance,
- *                   the adjoint (dot-product) identity and central finite
- *                   differences of the loss.
+ *                   preconditioner).  Pinned by an independent numpy
+ *                   restatement (np.roll shifts, einsum mixing), linearity,
+ *                   gauge covariance, the adjoint (dot-product) identity and
+ *                   central finite differences of the loss.
+ *   or_dot, or_norm2, or_true_residual  closed forms (sum conj(a) b); checked
+ *                   against numpy.vdot and the dense operator.
  */
 #include "qcu_oracle.h"
 
@@ -486,6 +489,7 @@ ocplx or_dot(const ocplx *a, const ocplx *b, long n) {
   return re + I_ * im;
 }
 double or_norm2(const ocplx *a, long n) { return creal(or_dot(a, a, n)); }
+void or_dot_out(ocplx *out, const ocplx *a, const ocplx *b, long n) { out[0] = or_dot(a, b, n); }
 
 /* ------------------------------------------------------------------------ */
 /* full-lattice operator M = A - kappa D  (A = 1 for Wilson)                 */

@@ -57,6 +57,8 @@ void or_mat_full(ocplx *out, const ocplx *in, const ocplx *U, const ocplx *Afull
 /* ---- BLAS-1 -------------------------------------------------------------- */
 ocplx or_dot(const ocplx *a, const ocplx *b, long n);
 double or_norm2(const ocplx *a, long n);
+/* out[0] = or_dot(a, b, n), for callers without C99 complex returns */
+void or_dot_out(ocplx *out, const ocplx *a, const ocplx *b, long n);
 
 /* ---- even-odd BiCGSTAB (include/lattice_bistabcg.h:221-340) ------------ */
 typedef struct {

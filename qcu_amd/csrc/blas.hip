@@ -356,7 +356,7 @@ __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, co
 // One block per system: fixed-order sum of the system's block partials, then
 // the epilogue.
 template <int NR>
-__global__ void __launch_bounds__(256) k_finish(const double *partials, const SegDesc *segs, int nseg, int bsz,
+__global__ void __launch_bounds__(1024) k_finish(const double *partials, const SegDesc *segs, int nseg, int bsz,
                                                 int kind, SysState *st, double *out) {
   const int sys = blockIdx.x;
   int seg = -1, r = 0, blk0 = 0, nb = 0, acc_b = 0;
@@ -373,6 +373,8 @@ __global__ void __launch_bounds__(256) k_finish(const double *partials, const Se
   }
   if (seg < 0) return;
   if (st && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO || kind == FK_RELIABLE) && st[sys].done) return;
+  // up to 8 independent loads in flight per thread: the partials are few
+  // (one per block) and the kernel is latency bound
   double acc4[4][NR];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -380,11 +382,16 @@ __global__ void __launch_bounds__(256) k_finish(const double *partials, const Se
     for (int k = 0; k < NR; ++k) acc4[u][k] = 0;
   const int bs = blockDim.x;
   int b = threadIdx.x;
-  for (; b + 3 * bs < nb; b += 4 * bs) {
+  for (; b + 7 * bs < nb; b += 8 * bs) {
+    double v[8][NR];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int k = 0; k < NR; ++k) acc4[u][k] += partials[((size_t)(blk0 + b + u * bs) * kMaxRhs + r) * NR + k];
+      for (int k = 0; k < NR; ++k) v[u][k] = partials[((size_t)(blk0 + b + u * bs) * kMaxRhs + r) * NR + k];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) acc4[u & 3][k] += v[u][k];
   }
   for (; b < nb; b += bs) {
 #pragma unroll
@@ -427,9 +434,9 @@ int finish(const BatchOp &op, const double *partials, int bsz, int nr, int kind,
   const SegDesc *d = op.dev_segs;
   const int nseg = (int)op.segs.size();
   switch (nr) {
-    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
-    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
-    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(nsys), dim3(256), 0, s, partials, d, nseg, bsz, kind, st, out); break;
+    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(nsys), dim3(1024), 0, s, partials, d, nseg, bsz, kind, st, out); break;
+    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(nsys), dim3(1024), 0, s, partials, d, nseg, bsz, kind, st, out); break;
+    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(nsys), dim3(1024), 0, s, partials, d, nseg, bsz, kind, st, out); break;
     default: set_error(QCU_ERR_ARG, "finish: bad reduction width"); return QCU_ERR_ARG;
   }
   QCU_HIP(hipGetLastError());

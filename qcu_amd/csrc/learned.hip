@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -48,6 +49,7 @@ struct NetLayerArgs {
   Geom g;
   int cin, cout;
   int parity0;    // first output parity (the grid's y index adds to it)
+  int ngroups;    // output-channel groups
   const SysState *st;
 };
 
@@ -70,7 +72,7 @@ struct TermGeo {
   float ph;
 };
 
-QCU_D TermGeo term_geo(const Geom &g, int i, int p, int t) {
+QCU_D TermGeo term_geo(const Geom &g, const Site &s, int i, int p, int t) {
   TermGeo r;
   r.j = i;
   r.jp = p;
@@ -81,7 +83,6 @@ QCU_D TermGeo term_geo(const Geom &g, int i, int p, int t) {
   r.ph = 1.f;
   if (!r.hop) return r;
   const int mu = t >> 1;
-  const Site s = decode(g, i, p);
   const int q = 1 - p;
   const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
   int j = i;
@@ -124,16 +125,23 @@ QCU_D void transport4(const cf (&u)[9], cf (&v)[12], int fwd, float ph) {
   }
 }
 
-// grid: x sites, y output parity, z output-channel group of CO channels;
-// CI: input channel count known at compile time (0: a.cin at run time)
+// grid: x = site blocks x output-channel groups (group fastest, so that the
+// groups of one site block run together and share its loads in L2; the
+// XCD-aware remap keeps neighbouring site blocks on one L2), y = output parity.
+// CO output channels per thread; CI: input channel count known at compile time
+// (0: a.cin at run time).  The input channel is the outer loop so that the
+// nine neighbour reads of one channel are close in time and mostly hit in L2.
 template <int CO, int CI>
 __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = remap_block(blockIdx.x, gridDim.x, 1);
+  const int sb = blk / a.ngroups;
+  const int og = (blk - sb * a.ngroups) * CO;
+  const int i = sb * blockDim.x + threadIdx.x;
   const int p = a.parity0 + blockIdx.y;
-  const int og = blockIdx.z * CO;
   if (i >= g.vh) return;
+  const Site site = decode(g, i, p);
   cf acc[CO][12];
 #pragma unroll
   for (int o = 0; o < CO; ++o)
@@ -141,15 +149,17 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
     for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
   const int cin = CI > 0 ? CI : a.cin, cout = a.cout;
 #pragma unroll 1
-  for (int t = 0; t < 9; ++t) {
-    const TermGeo tg = term_geo(g, i, p, t);
-    cf u[9];
-    if (tg.hop) LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+  for (int ci = 0; ci < cin; ++ci) {
 #pragma unroll 1
-    for (int ci = 0; ci < cin; ++ci) {
+    for (int t = 0; t < 9; ++t) {
+      const TermGeo tg = term_geo(g, site, i, p, t);
       cf v[12];
       SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
-      if (tg.hop) transport4(u, v, tg.fwd, tg.ph);
+      if (tg.hop) {
+        cf u[9];
+        LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+        transport4(u, v, tg.fwd, tg.ph);
+      }
 #pragma unroll
       for (int o = 0; o < CO; ++o) {
         if (og + o >= cout) break;
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nsite; k += gridDim.x * blockDim.x) {
     const int p = k >= g.vh ? 1 : 0;
     const int i = k - p * g.vh;
-    const TermGeo tg = term_geo(g, i, p, t);
+    const TermGeo tg = term_geo(g, decode(g, i, p), i, p, t);
     cf v[12];
     SpinorIO<P32>::load(chan_field(a.act, ci, tg.jp, g.vh), tg.j, v);
     if (tg.hop) {
@@ -444,6 +454,8 @@ struct Net : Precond {
   DevBuf tmp;                // odd FP32 vectors for training
   DevBuf red;                // reduction partials
   int adam_t = 0;
+  int co_max = 4;   // output channels per thread of the layer kernel (1, 2, 3, 4 or 8)
+  int block = 128;  // threads per block of the layer kernel
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
@@ -525,6 +537,15 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_HIP(hipMemset(m.p, 0, nw * 8));
   QCU_HIP(hipMemset(v.p, 0, nw * 8));
   adam_t = 0;
+  // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
+  if (const char *e = getenv("QCU_NET_SHAPE")) {
+    int c = 0, b = 0;
+    if (sscanf(e, "%d,%d", &c, &b) == 2 && (c == 1 || c == 2 || c == 3 || c == 4 || c == 8) &&
+        (b == 64 || b == 128 || b == 256)) {
+      co_max = c;
+      block = b;
+    }
+  }
   return 0;
 }
 
@@ -540,9 +561,11 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.cout = cout;
   a.parity0 = parity0;
   a.st = st;
-  const int bs = 128;
-  const int co = cout >= 4 ? 4 : cout;
-  dim3 grid((unsigned)ndiv(lat->g.vh, bs), nparity, (unsigned)ndiv(cout, co));
+  const int bs = block;
+  int co = cout < co_max ? cout : co_max;
+  if (co > 4 && co < 8) co = 4;
+  a.ngroups = (int)ndiv(cout, co);
+  dim3 grid((unsigned)(ndiv(lat->g.vh, bs) * a.ngroups), nparity, 1);
   const int ci = cin == 1 ? 1 : cin == 8 ? 8 : 0;
 #define QCU_NET_CASE(CO_, CI_) \
   if (co == CO_ && ci == CI_) hipLaunchKernelGGL((k_net_layer<CO_, CI_>), grid, dim3(bs), 0, s, a);
@@ -550,6 +573,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   QCU_NET_CASE(2, 0) QCU_NET_CASE(2, 1) QCU_NET_CASE(2, 8)
   QCU_NET_CASE(3, 0) QCU_NET_CASE(3, 1) QCU_NET_CASE(3, 8)
   QCU_NET_CASE(4, 0) QCU_NET_CASE(4, 1) QCU_NET_CASE(4, 8)
+  QCU_NET_CASE(8, 0) QCU_NET_CASE(8, 1) QCU_NET_CASE(8, 8)
 #undef QCU_NET_CASE
   QCU_HIP(hipGetLastError());
   return 0;

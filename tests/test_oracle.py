@@ -233,3 +233,24 @@ def test_bicgstab_reference_stopping_rule(orc):
     bh_norm2_est = np.vdot(b, b).real  # |b_hat| is of the same order
     assert info["schur_res"] ** 2 * bh_norm2_est < 1e-7
     assert info["iters"] < 1000
+
+
+def test_blas_closed_forms(orc):
+    """<a, b> = sum conj(a) b and |a|^2 against numpy on an odd length"""
+    a = orc.gauss((12345,), 1)
+    b = orc.gauss((12345,), 2)
+    assert abs(orc.dot(a, b) - np.vdot(a, b)) < 1e-12 * abs(np.vdot(a, b))
+    assert abs(orc.norm2(a) - np.vdot(a, a).real) < 1e-12 * np.vdot(a, a).real
+
+
+def test_true_residual_closed_form(orc):
+    """|b - M x| / |b| from the dense operator of the independent restatement"""
+    L, kappa, csw, tb = (4, 4, 4, 4), 0.124, 1.0, -1
+    U = orc.gauge_random(L, 0.3, 3)
+    x = orc.gauss((2, orc.vol(L) // 2, 4, 3), 4)
+    b = orc.gauss((2, orc.vol(L) // 2, 4, 3), 5)
+    Ul = indep.gauge_to_lexico(U, L)
+    mx = indep.fermion_to_cb(indep.mat_lexico(indep.fermion_to_lexico(x, L), Ul, L, kappa, kappa * csw, tb), L)
+    want = np.linalg.norm(b - mx) / np.linalg.norm(b)
+    got = orc.true_residual(x, b, U, L, kappa, csw, tb)
+    assert abs(got - want) < 1e-12 * want
