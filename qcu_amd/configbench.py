@@ -1,0 +1,139 @@
+"""Benchmarks of the smaller workloads (time-to-solution at 1e-10 true residual).
+
+    python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b]
+
+c1  plain clover BiCGSTAB FP64, 16^3x32, eps 0.3, csw 1.0, kappa 0.124, point
+    source (origin, spin 0, colour 0): iterations, time, FP64 hop GB/s.
+c2  24^3x48 clover, eps 0.25, csw 1.0, kappa 0.1245, Gaussian source: FP64,
+    and FP32 / FP16 inner solves at reliable-update drops 0.1, 0.01, 0.001,
+    with the error of each solution against the FP64 one.
+c3a 12 spin-colour point sources on the c2 lattice in one batch (FP32 inner),
+    against 12 single solves; the 12-RHS hop against the 1-RHS hop.
+c3b 16 Wilson lattices (4 each of 8^4, 12^4, 16^3x32, 20^3x40; eps 0.3,
+    kappa 0.124, Gaussian sources) in one batch against 16 single solves.
+
+Every solve must reach |b - M x| / |b| <= 1e-10 (FP64); one JSON line per
+workload.  Times are medians of 3 (2 for batches) after a warm-up solve.
+"""
+import json
+import sys
+import time
+
+import torch
+
+from .lattice import FP16, FP32, FP64, Batch, Lattice, gaussian, point_source, random_gauge
+
+# algorithmic bytes per output site of the hop, the clover inverse and the xpay operand
+HOP = {0: 8 * 192 + 8 * 96 + 192, 1: 8 * 96 + 8 * 48 + 96, 2: 8 * 52 + 8 * 24 + 52}
+CL = {0: 576, 1: 288, 2: 148}
+XP = {0: 192, 1: 96, 2: 52}
+TOL = 1e-10
+
+
+def timed(fn, n=3):
+    fn()
+    ts, r = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[n // 2], r
+
+
+def check(res):
+    res = res if isinstance(res, list) else [res]
+    for r in res:
+        if not (r.converged and r.true_res <= TOL):
+            raise RuntimeError(f"solve missed {TOL}: {r}")
+
+
+def c1():
+    L = (16, 16, 16, 32)
+    lat = Lattice(L, 0.124, 1.0, -1)
+    lat.load_gauge(random_gauge(L, 0.3, 11))
+    lat.tune(FP64)
+    b = point_source(L)
+    t, (x, r) = timed(lambda: lat.solve(b, tol=TOL, prec=FP64))
+    check(r)
+    ms = lat.bench_hop(FP64, True, True, 100)
+    return {"iters": r.iters, "tts_ms": t * 1e3, "true_res": r.true_res, "hop_fp64_clover_xpay_ms": ms,
+            "GBps": (HOP[0] + CL[0] + XP[0]) * lat.vh / ms / 1e6, "GFLOPs": 1320 * lat.vh / ms / 1e6}
+
+
+def c2():
+    L = (24, 24, 24, 48)
+    lat = Lattice(L, 0.1245, 1.0, -1)
+    lat.load_gauge(random_gauge(L, 0.25, 12))
+    for p in (FP64, FP32, FP16):
+        lat.tune(p)
+    b = gaussian((2, 48, 24, 24, 12, 4, 3), 13)
+    t64, (x64, r64) = timed(lambda: lat.solve(b, tol=TOL, prec=FP64))
+    check(r64)
+    d = {"fp64": {"iters": r64.iters, "tts_ms": t64 * 1e3, "true_res": r64.true_res}}
+    for name, p in (("fp32", FP32), ("fp16", FP16)):
+        for delta in (0.1, 0.01, 0.001):
+            t, (x, r) = timed(lambda: lat.solve(b, tol=TOL, prec=p, reliable_delta=delta))
+            check(r)
+            err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+            d[f"{name}_d{delta}"] = {"iters": r.iters, "reliable": r.reliable_updates, "tts_ms": t * 1e3,
+                                     "true_res": r.true_res, "rel_err_vs_fp64_solution": err}
+    for p in (FP64, FP32, FP16):
+        ms = lat.bench_hop(p, True, True, 100)
+        d[f"hop_p{p}_ms"] = ms
+        d[f"hop_p{p}_GBps"] = (HOP[p] + CL[p] + XP[p]) * lat.vh / ms / 1e6
+    return d
+
+
+def c3a():
+    L = (24, 24, 24, 48)
+    lat = Lattice(L, 0.1245, 1.0, -1)
+    lat.load_gauge(random_gauge(L, 0.25, 12))
+    lat.tune(FP32)
+    bs = [point_source(L, s, c) for s in range(4) for c in range(3)]
+    batch = Batch([lat], [12])
+    batch.tune(FP32)
+    tb, (xs, rs) = timed(lambda: batch.solve(bs, tol=TOL, prec=FP32, reliable_delta=0.01), 2)
+    check(rs)
+    ts, res = timed(lambda: [lat.solve(b, tol=TOL, prec=FP32, reliable_delta=0.01) for b in bs], 2)
+    err = max(float(torch.linalg.norm(x - y[0]) / torch.linalg.norm(y[0])) for x, y in zip(xs, res))
+    ms1 = lat.bench_hop(FP32, True, True, 50)
+    ms12 = batch.bench_hop(FP32, True, 50)
+    return {"batch_ms": tb * 1e3, "sequential_ms": ts * 1e3, "iters": [r.iters for r in rs],
+            "max_true_res": max(r.true_res for r in rs), "max_rel_diff_vs_single": err,
+            "hop_fp32_1rhs_ms": ms1, "hop_fp32_12rhs_ms": ms12, "hop_12rhs_per_rhs_ms": ms12 / 12}
+
+
+def c3b():
+    shapes = [(8, 8, 8, 8)] * 4 + [(12, 12, 12, 12)] * 4 + [(16, 16, 16, 32)] * 4 + [(20, 20, 20, 40)] * 4
+    lats, bs = [], []
+    for k, L in enumerate(shapes):
+        lat = Lattice(L, 0.124, 0.0, -1)
+        lat.load_gauge(random_gauge(L, 0.3, 100 + k))
+        lats.append(lat)
+        bs.append(gaussian((2, L[3], L[2], L[1], L[0] // 2, 4, 3), 200 + k))
+    batch = Batch(lats, [1] * 16)
+    for p in (FP64, FP32):
+        batch.tune(p)
+        for lat in lats:
+            lat.tune(p)
+    d = {}
+    for name, p, delta in (("fp64", FP64, 0.1), ("fp32", FP32, 0.01)):
+        tb, (xs, rs) = timed(lambda: batch.solve(bs, tol=TOL, prec=p, reliable_delta=delta), 2)
+        check(rs)
+        ts, _ = timed(lambda: [lat.solve(b, tol=TOL, prec=p, reliable_delta=delta) for lat, b in zip(lats, bs)], 2)
+        d[name] = {"batch_ms": tb * 1e3, "sequential_ms": ts * 1e3, "iters": [r.iters for r in rs],
+                   "max_true_res": max(r.true_res for r in rs)}
+    return d
+
+
+def main():
+    which = sys.argv[1:] or ["c1", "c2", "c3a", "c3b"]
+    fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b}
+    for w in which:
+        print(json.dumps({"workload": w, **fns[w]()}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -183,6 +183,173 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
   }
 }
 
+// ---- matrix-core layer -----------------------------------------------------
+// The channel and spin mixing of one layer is a complex GEMM per colour,
+//   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
+//   k = (term t, input channel i, spin s2),  n = site,
+// done as four real products on v_mfma_f32_16x16x4_f32 in FP32:
+//   Dr += Wr Vr,  Dr += Wi (-Vi),  Di += Wi Vr,  Di += Wr Vi.
+// Lane l of an MFMA step holds A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
+// and returns D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.  A K step is the four
+// spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
+// so no transport is repeated.  An M tile is four output channels times four
+// spins, so lane group g ends with all 12 colour-spin components of channel
+// 4 mt + g at its site.  Each wave owns 16 sites; the operands of the next
+// (input channel, term) step are loaded while the current one is multiplied.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// A operands of one layer: Apk[((t * cin + i) * MT + mt) * 2 + part][lane],
+// part 0 = Re W, 1 = Im W, row m = lane % 16 -> (o = 4 mt + m / 4, s = m % 4),
+// column k = lane / 16 -> s2
+__global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int mtiles) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = 9 * cin * mtiles * 2 * 64;
+  if (idx >= total) return;
+  const int lane = idx & 63;
+  int r = idx >> 6;
+  const int part = r & 1;
+  r >>= 1;
+  const int mt = r % mtiles;
+  r /= mtiles;
+  const int i = r % cin, t = r / cin;
+  const int ml = lane & 15, s2 = lane >> 4;
+  const int o = 4 * mt + (ml >> 2), sp = ml & 3;
+  float v = 0.f;
+  if (o < cout) {
+    const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + sp * 4 + s2];
+    v = part ? w.y : w.x;
+  }
+  Apk[idx] = v;
+}
+
+QCU_D void transport1(const cf (&u)[9], cf (&v)[3], int fwd, float ph) {
+  cf a[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    cf acc = mk(0.f, 0.f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (fwd) cmad(acc, u[r * 3 + c], v[c]);
+      else cmadj(acc, u[c * 3 + r], v[c]);
+    }
+    a[r] = mk(ph * acc.re, ph * acc.im);
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) v[r] = a[r];
+}
+
+// operands of one (input channel, term) step, loaded one step ahead
+template <int MT>
+struct MfmaStep {
+  float4 sa, sb;     // the two float4 slots holding the lane's spin
+  float4 l[3];       // rows 0-1 of the link
+  float wr[MT], wi[MT];
+  int hop, fwd;
+  float ph;
+};
+
+template <int MT>
+QCU_D void mfma_fetch(MfmaStep<MT> &f, const NetLayerArgs &a, const float *Apk, const Site &site, int i, int p,
+                      int gq, int lane, int q) {
+  const int ci = q / 9, t = q - 9 * ci;
+  const TermGeo tg = term_geo(a.g, site, i, p, t);
+  const SpinorPtr in = chan_field(a.in, ci, tg.jp, a.g.vh);
+  const float4 *b = (const float4 *)in.p + tg.j;
+  const int c0 = 3 * gq;
+  f.sa = b[(size_t)(c0 >> 1) * in.stride];
+  f.sb = b[(size_t)((c0 >> 1) + 1) * in.stride];
+  f.hop = tg.hop;
+  f.fwd = tg.fwd;
+  f.ph = tg.ph;
+  if (tg.hop) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f.l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
+  }
+  const float *A = Apk + (size_t)((t * a.cin + ci) * MT * 2) * 64 + lane;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    f.wr[mt] = A[(mt * 2) * 64];
+    f.wi[mt] = A[(mt * 2 + 1) * 64];
+  }
+}
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const float *Apk) {
+  if (a.st && a.st->done) return;
+  const Geom g = a.g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int blk = remap_block(blockIdx.x, gridDim.x, 1);
+  const int base = (blk * (int)(blockDim.x >> 6) + wave) * 16;
+  const int p = a.parity0 + blockIdx.y;
+  if (base >= g.vh) return;  // uniform over the wave
+  const int gq = lane >> 4;
+  int i = base + (lane & 15);
+  const bool ok = i < g.vh;
+  if (!ok) i = base;
+  const Site site = decode(g, i, p);
+  const float keep = ok ? 1.f : 0.f;
+  f4v dr[3][MT], di[3][MT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+    }
+  const int nq = 9 * a.cin;
+  MfmaStep<MT> nxt;
+  mfma_fetch<MT>(nxt, a, Apk, site, i, p, gq, lane, 0);
+#pragma unroll 1
+  for (int q = 0; q < nq; ++q) {
+    const MfmaStep<MT> cur = nxt;
+    if (q + 1 < nq) mfma_fetch<MT>(nxt, a, Apk, site, i, p, gq, lane, q + 1);
+    cf v[3];
+    if ((3 * gq) & 1) {
+      v[0] = mk(cur.sa.z, cur.sa.w);
+      v[1] = mk(cur.sb.x, cur.sb.y);
+      v[2] = mk(cur.sb.z, cur.sb.w);
+    } else {
+      v[0] = mk(cur.sa.x, cur.sa.y);
+      v[1] = mk(cur.sa.z, cur.sa.w);
+      v[2] = mk(cur.sb.x, cur.sb.y);
+    }
+    if (cur.hop) {
+      cf u[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        u[2 * k] = mk(cur.l[k].x, cur.l[k].y);
+        u[2 * k + 1] = mk(cur.l[k].z, cur.l[k].w);
+      }
+      recon_row2(u);
+      transport1(u, v, cur.fwd, cur.ph);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float br = keep * v[c].re, bi = keep * v[c].im;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wr[mt], br, dr[c][mt], 0, 0, 0);
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wi[mt], -bi, dr[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wi[mt], br, di[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wr[mt], bi, di[c][mt], 0, 0, 0);
+      }
+    }
+  }
+  // lane group gq holds channel 4 mt + gq: component (spin j, colour c) = (dr[c][mt][j], di[c][mt][j])
+  if (!ok) return;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int o = 4 * mt + gq;
+    if (o >= a.cout) continue;
+    cf out[12];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j], di[c][mt][j]);
+    SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
+  }
+}
+
 // weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
 //   sum over sites of both parities and colours of g_o(x)[s][c] conj((T_t a_ci)(x)[s2][c])
 struct NetWgradArgs {
@@ -454,7 +621,9 @@ struct Net : Precond {
   DevBuf tmp;                // odd FP32 vectors for training
   DevBuf red;                // reduction partials
   int adam_t = 0;
-  int co_max = 4;   // output channels per thread of the layer kernel (1, 2, 3, 4 or 8)
+  int co_max = 4;   // output channels per thread of the vector layer kernel (1, 2, 3, 4 or 8)
+  bool use_mfma = true;  // matrix-core layer kernel (QCU_NET_KERNEL=valu selects the vector one)
+  DevBuf Apk;       // A operands of the matrix-core kernel for one layer
   int block = 128;  // threads per block of the layer kernel
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
 
@@ -537,6 +706,8 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_HIP(hipMemset(m.p, 0, nw * 8));
   QCU_HIP(hipMemset(v.p, 0, nw * 8));
   adam_t = 0;
+  QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 4 * 2 * 64 * 4));
+  if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
   if (const char *e = getenv("QCU_NET_SHAPE")) {
     int c = 0, b = 0;
@@ -561,6 +732,19 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.cout = cout;
   a.parity0 = parity0;
   a.st = st;
+  if (use_mfma) {
+    const int mtiles = (cout + 3) / 4;
+    const int total = 9 * cin * mtiles * 2 * 64;
+    hipLaunchKernelGGL(k_net_pack, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, Wl, (float *)Apk.p, cin, cout,
+                       mtiles);
+    const int wsites = 16 * (block / 64);  // sites per block
+    dim3 grid((unsigned)ndiv(lat->g.vh, wsites), nparity, 1);
+    const float *ap = (const float *)Apk.p;
+    if (mtiles == 1) hipLaunchKernelGGL(k_net_layer_mfma<1>, grid, dim3(block), 0, s, a, ap);
+    else hipLaunchKernelGGL(k_net_layer_mfma<2>, grid, dim3(block), 0, s, a, ap);
+    QCU_HIP(hipGetLastError());
+    return 0;
+  }
   const int bs = block;
   int co = cout < co_max ? cout : co_max;
   if (co > 4 && co < 8) co = 4;
