@@ -35,6 +35,7 @@ CLOVER_BYTES = {0: 576, 1: 288, 2: 148}
 # the xpay epilogue of the Schur operator reads one more spinor
 XPAY_BYTES = {0: 192, 1: 96, 2: 52}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp16_clover_32x32x32x64_traffic.json")
+NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_32x32x32x64_traffic.json")
 
 
 def log(*a):
@@ -138,9 +139,13 @@ def main():
     log(f"learned+mixed: {t_learned * 1e3:.1f} ms {lres[-1]}")
 
     # roofline of the dominant kernel of the timed solve: the network's 8 -> 8
-    # channel layer (FP32 FMA, compute-bound), timed with HIP events
+    # channel layer (FP32 matrix cores, compute-bound), timed with HIP events
     ms_layer, fl_layer = net.bench_layer(1, iters=10)
     tflops = fl_layer / (ms_layer * 1e-3) / 1e12
+    net_traffic = None
+    if os.path.exists(NET_TRAFFIC):
+        with open(NET_TRAFFIC) as f:
+            net_traffic = json.load(f).get("hbm_bytes_per_launch")
     # the inner-precision clover hop with the xpay epilogue (HBM-bound)
     ms_hop = lat.bench_hop(prec=inner, clover=True, xpay=True, iters=100)
     alg = (HOP_BYTES[inner] + CLOVER_BYTES[inner] + XPAY_BYTES[inner]) * vh
@@ -189,9 +194,10 @@ def main():
         "training": {"seconds": round(train_s, 2), "sources": args.train_sources,
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
-        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer<4>)", "bound": "mfma",
-                     "compute": "FP32 vector FMA", "achieved": round(tflops, 2), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_TFLOPS, 3), "traffic": None,
+        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_mfma<2>)", "bound": "mfma",
+                     "compute": "FP32 MFMA v_mfma_f32_16x16x4_f32", "achieved": round(tflops, 2),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_TFLOPS, 3),
+                     "traffic": net_traffic,
                      "ms_per_launch": round(ms_layer, 4), "alg_flops_per_launch": fl_layer},
         "hop_roofline": {"kernel": f"hop {args.inner} clover xpay", "bound": "hbm",
                          "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
