@@ -194,8 +194,7 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
 // spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
 // so no transport is repeated.  An M tile is four output channels times four
 // spins, so lane group g ends with all 12 colour-spin components of channel
-// 4 mt + g at its site.  Each wave owns 16 sites; the operands of the next
-// (input channel, term) step are loaded while the current one is multiplied.
+// 4 mt + g at its site.  Each wave owns 16 sites.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // A operands of one layer: Apk[((t * cin + i) * MT + mt) * 2 + part][lane],
@@ -238,41 +237,9 @@ QCU_D void transport1(const cf (&u)[9], cf (&v)[3], int fwd, float ph) {
   for (int r = 0; r < 3; ++r) v[r] = a[r];
 }
 
-// operands of one (input channel, term) step, loaded one step ahead
-template <int MT>
-struct MfmaStep {
-  float4 sa, sb;     // the two float4 slots holding the lane's spin
-  float4 l[3];       // rows 0-1 of the link
-  float wr[MT], wi[MT];
-  int hop, fwd;
-  float ph;
-};
-
-template <int MT>
-QCU_D void mfma_fetch(MfmaStep<MT> &f, const NetLayerArgs &a, const float *Apk, const Site &site, int i, int p,
-                      int gq, int lane, int q) {
-  const int ci = q / 9, t = q - 9 * ci;
-  const TermGeo tg = term_geo(a.g, site, i, p, t);
-  const SpinorPtr in = chan_field(a.in, ci, tg.jp, a.g.vh);
-  const float4 *b = (const float4 *)in.p + tg.j;
-  const int c0 = 3 * gq;
-  f.sa = b[(size_t)(c0 >> 1) * in.stride];
-  f.sb = b[(size_t)((c0 >> 1) + 1) * in.stride];
-  f.hop = tg.hop;
-  f.fwd = tg.fwd;
-  f.ph = tg.ph;
-  if (tg.hop) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) f.l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
-  }
-  const float *A = Apk + (size_t)((t * a.cin + ci) * MT * 2) * 64 + lane;
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    f.wr[mt] = A[(mt * 2) * 64];
-    f.wi[mt] = A[(mt * 2 + 1) * 64];
-  }
-}
-
+// The term is the outer loop: the link of each term is loaded and
+// reconstructed once and the input channels stream through it; the next
+// channel's spinor and weights are loaded one step ahead.
 template <int MT>
 __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const float *Apk) {
   if (a.st && a.st->done) return;
@@ -288,6 +255,7 @@ __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const fl
   if (!ok) i = base;
   const Site site = decode(g, i, p);
   const float keep = ok ? 1.f : 0.f;
+  const int odd = (3 * gq) & 1, slot = (3 * gq) >> 1;
   f4v dr[3][MT], di[3][MT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -296,46 +264,74 @@ __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const fl
       dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
       di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
-  const int nq = 9 * a.cin;
-  MfmaStep<MT> nxt;
-  mfma_fetch<MT>(nxt, a, Apk, site, i, p, gq, lane, 0);
+  const int cin = a.cin;
 #pragma unroll 1
-  for (int q = 0; q < nq; ++q) {
-    const MfmaStep<MT> cur = nxt;
-    if (q + 1 < nq) mfma_fetch<MT>(nxt, a, Apk, site, i, p, gq, lane, q + 1);
-    cf v[3];
-    if ((3 * gq) & 1) {
-      v[0] = mk(cur.sa.z, cur.sa.w);
-      v[1] = mk(cur.sb.x, cur.sb.y);
-      v[2] = mk(cur.sb.z, cur.sb.w);
-    } else {
-      v[0] = mk(cur.sa.x, cur.sa.y);
-      v[1] = mk(cur.sa.z, cur.sa.w);
-      v[2] = mk(cur.sb.x, cur.sb.y);
-    }
-    if (cur.hop) {
-      cf u[9];
+  for (int t = 0; t < 9; ++t) {
+    const TermGeo tg = term_geo(g, site, i, p, t);
+    cf u[9];
+    if (tg.hop) {
+      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        u[2 * k] = mk(cur.l[k].x, cur.l[k].y);
-        u[2 * k + 1] = mk(cur.l[k].z, cur.l[k].w);
-      }
-      recon_row2(u);
-      transport1(u, v, cur.fwd, cur.ph);
+      for (int k = 0; k < 9; ++k) u[k] = mk(tg.ph * u[k].re, tg.ph * u[k].im);
     }
+    const float *Ab = Apk + (size_t)(t * cin * MT * 2) * 64 + lane;
+    auto spin_ptr = [&](int ci) {
+      const SpinorPtr in = chan_field(a.in, ci, tg.jp, g.vh);
+      return (const float4 *)in.p + tg.j + (size_t)slot * in.stride;
+    };
+    const size_t stride = (size_t)g.vh;
+    const float4 *sp = spin_ptr(0);
+    float4 na = sp[0], nb = sp[stride];
+    float nwr[MT], nwi[MT];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float br = keep * v[c].re, bi = keep * v[c].im;
+    for (int mt = 0; mt < MT; ++mt) {
+      nwr[mt] = Ab[(mt * 2) * 64];
+      nwi[mt] = Ab[(mt * 2 + 1) * 64];
+    }
+#pragma unroll 1
+    for (int ci = 0; ci < cin; ++ci) {
+      const float4 ca = na, cb = nb;
+      float wr[MT], wi[MT];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wr[mt], br, dr[c][mt], 0, 0, 0);
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wi[mt], -bi, dr[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wi[mt], br, di[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.wr[mt], bi, di[c][mt], 0, 0, 0);
+        wr[mt] = nwr[mt];
+        wi[mt] = nwi[mt];
+      }
+      if (ci + 1 < cin) {
+        const float4 *q = spin_ptr(ci + 1);
+        na = q[0];
+        nb = q[stride];
+        const float *A = Ab + (size_t)((ci + 1) * MT * 2) * 64;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          nwr[mt] = A[(mt * 2) * 64];
+          nwi[mt] = A[(mt * 2 + 1) * 64];
+        }
+      }
+      cf v[3];
+      if (odd) {
+        v[0] = mk(ca.z, ca.w);
+        v[1] = mk(cb.x, cb.y);
+        v[2] = mk(cb.z, cb.w);
+      } else {
+        v[0] = mk(ca.x, ca.y);
+        v[1] = mk(ca.z, ca.w);
+        v[2] = mk(cb.x, cb.y);
+      }
+      if (tg.hop) transport1(u, v, tg.fwd, 1.f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float br = keep * v[c].re, bi = keep * v[c].im;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], br, dr[c][mt], 0, 0, 0);
+          dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], -bi, dr[c][mt], 0, 0, 0);
+          di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], br, di[c][mt], 0, 0, 0);
+          di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], bi, di[c][mt], 0, 0, 0);
+        }
       }
     }
   }
-  // lane group gq holds channel 4 mt + gq: component (spin j, colour c) = (dr[c][mt][j], di[c][mt][j])
   if (!ok) return;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
