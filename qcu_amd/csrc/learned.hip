@@ -241,7 +241,7 @@ QCU_D void transport1(const cf (&u)[9], cf (&v)[3], int fwd, float ph) {
 // reconstructed once and the input channels stream through it; the next
 // channel's spinor and weights are loaded one step ahead.
 template <int MT>
-__global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const float *Apk) {
+QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -344,6 +344,11 @@ __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const fl
       for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j], di[c][mt][j]);
     SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
   }
+}
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const float *Apk) {
+  net_layer_mfma_body<MT>(a, Apk);
 }
 
 // weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
@@ -728,7 +733,9 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.cout = cout;
   a.parity0 = parity0;
   a.st = st;
-  if (use_mfma) {
+  // a single output channel fills a quarter of the matrix-core tiles while the
+  // transports cost the same, so that layer runs on the vector units
+  if (use_mfma && cout > 1) {
     const int mtiles = (cout + 3) / 4;
     const int total = 9 * cin * mtiles * 2 * 64;
     hipLaunchKernelGGL(k_net_pack, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, Wl, (float *)Apk.p, cin, cout,
