@@ -49,6 +49,7 @@ struct NetLayerArgs {
   Geom g;
   int cin, cout;
   int parity0;    // first output parity (the grid's y index adds to it)
+  int zero_par;   // parity on which the input is known to be zero (-1: none)
   int ngroups;    // output-channel groups
   const SysState *st;
 };
@@ -148,10 +149,11 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
   const int cin = CI > 0 ? CI : a.cin, cout = a.cout;
+  const int t_lo = (a.zero_par == 1 - p) ? 8 : 0, t_hi = (a.zero_par == p) ? 8 : 9;
 #pragma unroll 1
   for (int ci = 0; ci < cin; ++ci) {
 #pragma unroll 1
-    for (int t = 0; t < 9; ++t) {
+    for (int t = t_lo; t < t_hi; ++t) {
       const TermGeo tg = term_geo(g, site, i, p, t);
       cf v[12];
       SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
@@ -238,8 +240,9 @@ QCU_D void transport1(const cf (&u)[9], cf (&v)[3], int fwd, float ph) {
 }
 
 // The term is the outer loop: the link of each term is loaded and
-// reconstructed once and the input channels stream through it; the next
-// channel's spinor and weights are loaded one step ahead.
+// reconstructed once and the input channels stream through it.  The operands
+// of the next (term, channel) step -- spinor, weights, and at a term boundary
+// the next link -- are loaded one step ahead.
 template <int MT>
 QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   if (a.st && a.st->done) return;
@@ -256,6 +259,7 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   const Site site = decode(g, i, p);
   const float keep = ok ? 1.f : 0.f;
   const int odd = (3 * gq) & 1, slot = (3 * gq) >> 1;
+  const size_t vh = (size_t)g.vh;
   f4v dr[3][MT], di[3][MT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -265,71 +269,102 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
       di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
   const int cin = a.cin;
-#pragma unroll 1
-  for (int t = 0; t < 9; ++t) {
-    const TermGeo tg = term_geo(g, site, i, p, t);
-    cf u[9];
-    if (tg.hop) {
-      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+  // terms that read the zero parity are skipped: hop terms read parity 1 - p,
+  // the site term parity p
+  int t_lo = 0, t_hi = 9;
+  if (a.zero_par == 1 - p) t_lo = 8;
+  if (a.zero_par == p) t_hi = 8;
+  const int q0 = t_lo * cin, nq = t_hi * cin;
+  // the lane's spin (two float4 slots) of channel ci at the neighbour of term geometry tg
+  auto spin_ptr = [&](const TermGeo &tg, int ci) {
+    const SpinorPtr in = chan_field(a.in, ci, tg.jp, g.vh);
+    return (const float4 *)in.p + tg.j + (size_t)slot * vh;
+  };
+  auto link_raw = [&](const TermGeo &tg, float4 (&l)[3]) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) u[k] = mk(tg.ph * u[k].re, tg.ph * u[k].im);
+    for (int k = 0; k < 3; ++k) l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
+  };
+  // operands of the first step
+  TermGeo ntg = term_geo(g, site, i, p, t_lo);
+  float4 nl[3];
+  if (ntg.hop) link_raw(ntg, nl);
+  const float4 *sp = spin_ptr(ntg, 0);
+  float4 na = sp[0], nb = sp[vh];
+  float nwr[MT], nwi[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    nwr[mt] = Apk[(size_t)(q0 * MT * 2 + mt * 2) * 64 + lane];
+    nwi[mt] = Apk[(size_t)(q0 * MT * 2 + mt * 2 + 1) * 64 + lane];
+  }
+  TermGeo tg = ntg;
+  cf u[9];
+  int t = t_lo, ci = 0;
+#pragma unroll 1
+  for (int q = q0; q < nq; ++q) {
+    if (ci == 0) {  // a new term: its link was loaded one step ahead
+      tg = ntg;
+      if (tg.hop) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          u[2 * k] = mk(nl[k].x, nl[k].y);
+          u[2 * k + 1] = mk(nl[k].z, nl[k].w);
+        }
+        recon_row2(u);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) u[k] = mk(tg.ph * u[k].re, tg.ph * u[k].im);
+      }
     }
-    const float *Ab = Apk + (size_t)(t * cin * MT * 2) * 64 + lane;
-    auto spin_ptr = [&](int ci) {
-      const SpinorPtr in = chan_field(a.in, ci, tg.jp, g.vh);
-      return (const float4 *)in.p + tg.j + (size_t)slot * in.stride;
-    };
-    const size_t stride = (size_t)g.vh;
-    const float4 *sp = spin_ptr(0);
-    float4 na = sp[0], nb = sp[stride];
-    float nwr[MT], nwi[MT];
+    const float4 ca = na, cb = nb;
+    float wr[MT], wi[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      nwr[mt] = Ab[(mt * 2) * 64];
-      nwi[mt] = Ab[(mt * 2 + 1) * 64];
+      wr[mt] = nwr[mt];
+      wi[mt] = nwi[mt];
     }
-#pragma unroll 1
-    for (int ci = 0; ci < cin; ++ci) {
-      const float4 ca = na, cb = nb;
-      float wr[MT], wi[MT];
+    // operands of step q + 1
+    if (q + 1 < nq) {
+      const float4 *np;
+      if (ci + 1 < cin) {
+        np = spin_ptr(tg, ci + 1);
+      } else {
+        ntg = term_geo(g, site, i, p, t + 1);
+        if (ntg.hop) link_raw(ntg, nl);
+        np = spin_ptr(ntg, 0);
+      }
+      na = np[0];
+      nb = np[vh];
+      const float *A = Apk + (size_t)((q + 1) * MT * 2) * 64 + lane;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        wr[mt] = nwr[mt];
-        wi[mt] = nwi[mt];
+        nwr[mt] = A[(mt * 2) * 64];
+        nwi[mt] = A[(mt * 2 + 1) * 64];
       }
-      if (ci + 1 < cin) {
-        const float4 *q = spin_ptr(ci + 1);
-        na = q[0];
-        nb = q[stride];
-        const float *A = Ab + (size_t)((ci + 1) * MT * 2) * 64;
+    }
+    cf v[3];
+    if (odd) {
+      v[0] = mk(ca.z, ca.w);
+      v[1] = mk(cb.x, cb.y);
+      v[2] = mk(cb.z, cb.w);
+    } else {
+      v[0] = mk(ca.x, ca.y);
+      v[1] = mk(ca.z, ca.w);
+      v[2] = mk(cb.x, cb.y);
+    }
+    if (tg.hop) transport1(u, v, tg.fwd, 1.f);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          nwr[mt] = A[(mt * 2) * 64];
-          nwi[mt] = A[(mt * 2 + 1) * 64];
-        }
-      }
-      cf v[3];
-      if (odd) {
-        v[0] = mk(ca.z, ca.w);
-        v[1] = mk(cb.x, cb.y);
-        v[2] = mk(cb.z, cb.w);
-      } else {
-        v[0] = mk(ca.x, ca.y);
-        v[1] = mk(ca.z, ca.w);
-        v[2] = mk(cb.x, cb.y);
-      }
-      if (tg.hop) transport1(u, v, tg.fwd, 1.f);
+    for (int c = 0; c < 3; ++c) {
+      const float br = keep * v[c].re, bi = keep * v[c].im;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float br = keep * v[c].re, bi = keep * v[c].im;
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], br, dr[c][mt], 0, 0, 0);
-          dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], -bi, dr[c][mt], 0, 0, 0);
-          di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], br, di[c][mt], 0, 0, 0);
-          di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], bi, di[c][mt], 0, 0, 0);
-        }
+      for (int mt = 0; mt < MT; ++mt) {
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], br, dr[c][mt], 0, 0, 0);
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], -bi, dr[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], br, di[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], bi, di[c][mt], 0, 0, 0);
       }
+    }
+    if (++ci == cin) {
+      ci = 0;
+      ++t;
     }
   }
   if (!ok) return;
@@ -637,8 +672,9 @@ struct Net : Precond {
     return s;
   }
   int layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
-            const SysState *st, hipStream_t s);
-  int forward(const SysState *st, hipStream_t s, bool odd_only_last);
+            const SysState *st, hipStream_t s, int zero_par = -1);
+  // zero_par0: parity on which the network input is zero (-1: none)
+  int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
   int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in);
   int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
 };
@@ -722,7 +758,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
 }
 
 int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
-               const SysState *st, hipStream_t s) {
+               const SysState *st, hipStream_t s, int zero_par) {
   NetLayerArgs a;
   a.in = in;
   a.out = out;
@@ -732,6 +768,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.cin = cin;
   a.cout = cout;
   a.parity0 = parity0;
+  a.zero_par = zero_par;
   a.st = st;
   // a single output channel fills a quarter of the matrix-core tiles while the
   // transports cost the same, so that layer runs on the vector units
@@ -767,11 +804,12 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
 }
 
 // act[0] holds the input; fills act[1..nl] (act[nl] = output)
-int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last) {
+int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0) {
   for (int k = 0; k < nl; ++k) {
     const bool last = k == nl - 1;
     const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
-    QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s));
+    QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s,
+                  k == 0 ? zero_par0 : -1));
   }
   return 0;
 }
@@ -835,7 +873,7 @@ int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
     default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
   }
   QCU_HIP(hipGetLastError());
-  QCU_TRY(forward(st, S.stream, true));
+  QCU_TRY(forward(st, S.stream, true, 0));
   const SpinorPtr an = act_ptr(nl);
   switch (prec) {
     case P64: hipLaunchKernelGGL(k_net_extract<P64>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
@@ -903,7 +941,7 @@ struct NetTrainer {
     // y = K b / |b|
     hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), odd(0), vh, nullptr);
     QCU_HIP(hipGetLastError());
-    QCU_TRY(net.forward(nullptr, s, false));
+    QCU_TRY(net.forward(nullptr, s, false, 0));
     hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, odd(1), net.act_ptr(net.nl), vh, nullptr);
     QCU_HIP(hipGetLastError());
     QCU_TRY(mhat32(lat, odd(2), odd(1), odd(3), s));
@@ -1009,7 +1047,7 @@ int qcuNetForward(QcuNetHandle h, void *out, const void *in) {
   hipLaunchKernelGGL(k_full_in, dim3((unsigned)ndiv(n0, 256)), dim3(256), 0, s, net.act_ptr(0),
                      (const double2 *)in, net.chan[0], vh);
   QCU_HIP(hipGetLastError());
-  QCU_TRY(net.forward(nullptr, s, false));
+  QCU_TRY(net.forward(nullptr, s, false, -1));
   const long nn = (long)net.chan[net.nl] * 2 * vh;
   hipLaunchKernelGGL(k_full_out, dim3((unsigned)ndiv(nn, 256)), dim3(256), 0, s, (double2 *)out,
                      net.act_ptr(net.nl), net.chan[net.nl], vh);
@@ -1029,7 +1067,7 @@ int qcuNetBackward(QcuNetHandle h, void *grad_in, const void *grad_out, const vo
   const long n0 = (long)net.chan[0] * 2 * vh;
   hipLaunchKernelGGL(k_full_in, dim3((unsigned)ndiv(n0, 256)), dim3(256), 0, s, net.act_ptr(0),
                      (const double2 *)in, net.chan[0], vh);
-  QCU_TRY(net.forward(nullptr, s, false));
+  QCU_TRY(net.forward(nullptr, s, false, -1));
   SpinorPtr go;
   go.p = net.gbuf[1].p;
   go.norm = nullptr;
@@ -1116,7 +1154,7 @@ int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
   QCU_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
   QCU_HIP(hipGetLastError());
-  QCU_TRY(net.forward(nullptr, s, true));
+  QCU_TRY(net.forward(nullptr, s, true, 0));
   SpinorPtr o1 = net.act_ptr(net.nl);
   o1.p = (char *)o1.p + (size_t)vh * 96;  // channel 0, parity 1
   hipLaunchKernelGGL(k_store_odd, dim3(nb), dim3(bs), 0, s, (double2 *)out, o1, vh);
