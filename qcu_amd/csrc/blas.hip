@@ -136,7 +136,7 @@ __global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v,
     zero12<Px>(zx);
     SpinorIO<Px>::store(view<Px>(x, c), c.t, zx);
   }
-  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 // p = r + beta (p - omega v)
@@ -216,7 +216,7 @@ __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p,
     }
     SpinorIO<P>::store(view<P>(r, c), c.t, ss);
   }
-  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 // two-vector reductions: <a, b> with |a|^2 (omega of the preconditioned method)
@@ -239,7 +239,7 @@ __global__ void k_dot2(SpinorPtr a_, SpinorPtr b_, const SegDesc *segs, int nseg
                           : (double)b[k].re * b[k].re + (double)b[k].im * b[k].im;
     }
   }
-  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<3>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 // y = a x + b y (all systems)
@@ -275,7 +275,7 @@ __global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, const SegDesc *segs, int ns
     }
     SpinorIO<P64>::store(view<P64>(r, c), c.t, rr);
   }
-  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 template <int P>
@@ -289,7 +289,7 @@ __global__ void k_norm2(SpinorPtr x, const SegDesc *segs, int nseg, double *part
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[0] += (double)xx[k].re * xx[k].re + (double)xx[k].im * xx[k].im;
   }
-  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 // out = C (x + a y) in FP64; a = kappa of the segment when a_is_kappa;
@@ -349,7 +349,7 @@ __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, co
       acc[1] += norm2(bb[k]);
     }
   }
-  block_partial_rhs<2>(acc, partials, blockIdx.x, c.t0, c.nrhs);
+  block_partial_rhs<2>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
 // ---------------------------------------------------------------- finishing kernel
@@ -373,8 +373,9 @@ __global__ void __launch_bounds__(1024) k_finish(const double *partials, const S
   }
   if (seg < 0) return;
   if (st && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO || kind == FK_RELIABLE) && st[sys].done) return;
-  // up to 8 independent loads in flight per thread: the partials are few
-  // (one per block) and the kernel is latency bound
+  // this system's partials are contiguous: slots r * pcap + blk0 .. + nb - 1;
+  // up to 8 independent loads in flight per thread (the kernel is latency bound)
+  const double *part = partials + ((size_t)r * segs[seg].pcap + blk0) * NR;
   double acc4[4][NR];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -387,7 +388,7 @@ __global__ void __launch_bounds__(1024) k_finish(const double *partials, const S
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int k = 0; k < NR; ++k) v[u][k] = partials[((size_t)(blk0 + b + u * bs) * kMaxRhs + r) * NR + k];
+      for (int k = 0; k < NR; ++k) v[u][k] = part[(size_t)(b + u * bs) * NR + k];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
@@ -395,7 +396,7 @@ __global__ void __launch_bounds__(1024) k_finish(const double *partials, const S
   }
   for (; b < nb; b += bs) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) acc4[0][k] += partials[((size_t)(blk0 + b) * kMaxRhs + r) * NR + k];
+    for (int k = 0; k < NR; ++k) acc4[0][k] += part[(size_t)b * NR + k];
   }
   double acc[NR];
 #pragma unroll

@@ -241,9 +241,9 @@ __global__ void __launch_bounds__(256) k_hop(HopKArgs a) {
   }
   if (RED == RED_DOT_Y_OUT) {
     double v[2] = {red[0], red[1]};
-    block_partial_rhs<2>(v, a.partials, blk, bs.t0, nrhs);
+    block_partial_rhs<2>(v, a.partials, blk, bs.t0, nrhs, sd.pcap);
   } else if (RED == RED_OUT_X) {
-    block_partial_rhs<3>(red, a.partials, blk, bs.t0, nrhs);
+    block_partial_rhs<3>(red, a.partials, blk, bs.t0, nrhs, sd.pcap);
   }
 }
 

@@ -26,7 +26,7 @@ struct SegDesc {
   int nrhs;      // interleaved right-hand sides
   int sys0;      // solver state index of rhs 0
   int nthr;      // vh * nrhs virtual sites
-  int pad0;
+  int pcap;      // block-partial slots per right-hand side (the same for every segment of a batch)
   long long voff;  // virtual-site offset of this segment in batch vectors
   double kappa;
   LinkPtr U[3];          // links per precision
@@ -69,13 +69,14 @@ QCU_HD SpinorPtr seg_view(const SpinorPtr &f, int prec, long long voff, int nthr
   return v;
 }
 
-// Per-system block partials: partials[(blk * kMaxRhs + r) * NR + k].  Threads
-// of a block belong to one segment; rhs r of thread tid is (t0 + tid) % nrhs.
+// Per-system block partials: partials[(r * pcap + blk) * NR + k], so that the
+// partials of one system are contiguous for the finishing kernel.  Threads of
+// a block belong to one segment; rhs r of thread tid is (t0 + tid) % nrhs.
 // Deterministic: a fixed-order sum per rhs.
 template <int NR>
-QCU_D void block_partial_rhs(const double (&v)[NR], double *partials, int blk, int t0, int nrhs) {
+QCU_D void block_partial_rhs(const double (&v)[NR], double *partials, int blk, int t0, int nrhs, int pcap) {
   if (nrhs == 1) {
-    block_partial<NR>(v, partials, blk * kMaxRhs);
+    block_partial<NR>(v, partials, blk);
     return;
   }
   __shared__ double sh[256][NR];
@@ -92,7 +93,7 @@ QCU_D void block_partial_rhs(const double (&v)[NR], double *partials, int blk, i
 #pragma unroll
       for (int k = 0; k < NR; ++k) acc[k] += sh[j][k];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) partials[((size_t)blk * kMaxRhs + r) * NR + k] = acc[k];
+    for (int k = 0; k < NR; ++k) partials[((size_t)r * pcap + blk) * NR + k] = acc[k];
   }
 }
 

@@ -84,6 +84,10 @@ int BatchOp::build(const std::vector<const Lattice *> &ls, const std::vector<int
       if (!l->has_clover()) return false;
     return true;
   }(), QCU_ERR_ARG, "batch: mixing Wilson and clover lattices is not supported");
+  // block-partial slots per right-hand side: the block count at the smallest
+  // block size a reducing kernel uses (64), the same stride for every segment
+  const int pcap = max_blocks(64) + 1;
+  for (SegDesc &d : segs) d.pcap = pcap;
   QCU_TRY(dseg.alloc(segs.size() * sizeof(SegDesc)));
   dev_segs = (SegDesc *)dseg.p;
   QCU_HIP(hipMemcpyAsync(dev_segs, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
