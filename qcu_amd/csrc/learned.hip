@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
 //   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
 //   k = (term t, input channel i, spin s2),  n = site,
 // done as four real products on v_mfma_f32_16x16x4_f32 in FP32:
-//   Dr += Wr Vr,  Dr += Wi (-Vi),  Di += Wi Vr,  Di += Wr Vi.
+//   Dr += Wr Vr,  Dn += Wi Vi,  Di += Wi Vr,  Di += Wr Vi,  Re D = Dr - Dn.
 // Lane l of an MFMA step holds A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
 // and returns D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.  A K step is the four
 // spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
@@ -257,15 +257,16 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   const bool ok = i < g.vh;
   if (!ok) i = base;
   const Site site = decode(g, i, p);
-  const float keep = ok ? 1.f : 0.f;
   const int odd = (3 * gq) & 1, slot = (3 * gq) >> 1;
   const size_t vh = (size_t)g.vh;
-  f4v dr[3][MT], di[3][MT];
+  // Re D = Wr Vr - Wi Vi is kept as two sums (dr, dn) subtracted at the end
+  f4v dr[3][MT], dn[3][MT], di[3][MT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      dn[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
       di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
   const int cin = a.cin;
@@ -352,12 +353,14 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
     }
     if (tg.hop) transport1(u, v, tg.fwd, 1.f);
 #pragma unroll
+    // lanes past the last site hold a copy of a valid site: their columns of D
+    // are computed and never stored
     for (int c = 0; c < 3; ++c) {
-      const float br = keep * v[c].re, bi = keep * v[c].im;
+      const float br = v[c].re, bi = v[c].im;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], br, dr[c][mt], 0, 0, 0);
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], -bi, dr[c][mt], 0, 0, 0);
+        dn[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], bi, dn[c][mt], 0, 0, 0);
         di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], br, di[c][mt], 0, 0, 0);
         di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], bi, di[c][mt], 0, 0, 0);
       }
@@ -376,7 +379,7 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j], di[c][mt][j]);
+      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j] - dn[c][mt][j], di[c][mt][j]);
     SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
   }
 }

@@ -145,3 +145,15 @@ def test_trained_preconditioner_reduces_iterations(orc):
     assert r0.converged and r1.converged and t0 <= 1e-10 and t1 <= 1e-10
     assert not r1.fallback
     assert r1.iters < r0.iters
+
+
+def test_vector_layer_kernel_matches_oracle(orc, monkeypatch):
+    """the vector-FMA layer kernel (kept as a cross-check of the matrix-core one)"""
+    monkeypatch.setenv("QCU_NET_KERNEL", "valu")
+    L, chan, tb = (8, 4, 4, 8), (1, 8, 8, 8, 1), -1
+    U, lat, net, W = _setup(orc, L, chan, tb, 150)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 151)
+    ref, _ = orc.net_forward(x, W, chan, U, L, tb)
+    e = rel(host(net.forward(dev(x))).ravel(), ref)
+    log(f"vector layer kernel: forward {e:.1e}")
+    assert e < 2e-5
