@@ -1,6 +1,6 @@
 """Benchmarks of the smaller workloads (time-to-solution at 1e-10 true residual).
 
-    python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b]
+    python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b] [c4]
 
 c1  plain clover BiCGSTAB FP64, 16^3x32, eps 0.3, csw 1.0, kappa 0.124, point
     source (origin, spin 0, colour 0): iterations, time, FP64 hop GB/s.
@@ -11,6 +11,9 @@ c3a 12 spin-colour point sources on the c2 lattice in one batch (FP32 inner),
     against 12 single solves; the 12-RHS hop against the 1-RHS hop.
 c3b 16 Wilson lattices (4 each of 8^4, 12^4, 16^3x32, 20^3x40; eps 0.3,
     kappa 0.124, Gaussian sources) in one batch against 16 single solves.
+c4  32^3x64 clover, eps 0.2, csw 1.0, kappa 0.1248, Gaussian source: plain
+    FP64 and mixed FP16 (reliable-update drop 0.01) solves, no network (the
+    solver part of the headline benchmark, e.g. for profiling).
 
 Every solve must reach |b - M x| / |b| <= 1e-10 (FP64); one JSON line per
 workload.  Times are medians of 3 (2 for batches) after a warm-up solve.
@@ -128,9 +131,24 @@ def c3b():
     return d
 
 
+def c4():
+    L = (32, 32, 32, 64)
+    lat = Lattice(L, 0.1248, 1.0, -1)
+    lat.load_gauge(random_gauge(L, 0.2, 20241))
+    b = gaussian((2, 64, 32, 32, 16, 4, 3), 777)
+    for p in (FP64, FP16):
+        lat.tune(p)
+    d = {}
+    for name, p, delta in (("fp64", FP64, 0.1), ("fp16", FP16, 0.01)):
+        t, (x, r) = timed(lambda: lat.solve(b, tol=TOL, prec=p, reliable_delta=delta))
+        check(r)
+        d[name] = {"iters": r.iters, "reliable": r.reliable_updates, "tts_ms": t * 1e3, "true_res": r.true_res}
+    return d
+
+
 def main():
     which = sys.argv[1:] or ["c1", "c2", "c3a", "c3b"]
-    fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b}
+    fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b, "c4": c4}
     for w in which:
         print(json.dumps({"workload": w, **fns[w]()}), flush=True)
 
