@@ -36,6 +36,15 @@ CLOVER_BYTES = {0: 576, 1: 288, 2: 148}
 XPAY_BYTES = {0: 192, 1: 96, 2: 52}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp16_clover_32x32x32x64_traffic.json")
 NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_32x32x32x64_traffic.json")
+HOP64_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_clover_32x32x32x64_traffic.json")
+
+
+def traffic_of(path):
+    """HBM bytes per launch recorded from the PMC runs of profiles/ (None if absent)"""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
 def log(*a):
@@ -142,18 +151,13 @@ def main():
     # channel layer (FP32 matrix cores, compute-bound), timed with HIP events
     ms_layer, fl_layer = net.bench_layer(1, iters=10)
     tflops = fl_layer / (ms_layer * 1e-3) / 1e12
-    net_traffic = None
-    if os.path.exists(NET_TRAFFIC):
-        with open(NET_TRAFFIC) as f:
-            net_traffic = json.load(f).get("hbm_bytes_per_launch")
+    net_traffic = traffic_of(NET_TRAFFIC)
     # the inner-precision clover hop with the xpay epilogue (HBM-bound)
     ms_hop = lat.bench_hop(prec=inner, clover=True, xpay=True, iters=100)
     alg = (HOP_BYTES[inner] + CLOVER_BYTES[inner] + XPAY_BYTES[inner]) * vh
     achieved = alg / (ms_hop * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = traffic_of(PROFILE_TRAFFIC)
+    traffic64 = traffic_of(HOP64_TRAFFIC)
     ms64 = lat.bench_hop(prec=FP64, clover=True, xpay=True, iters=50)
     alg64 = (HOP_BYTES[0] + CLOVER_BYTES[0] + XPAY_BYTES[0]) * vh
 
@@ -202,9 +206,12 @@ def main():
         "hop_roofline": {"kernel": f"hop {args.inner} clover xpay", "bound": "hbm",
                          "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,
+                         "hbm_GBps": None if traffic is None else round(traffic / (ms_hop * 1e-3) / 1e9, 1),
                          "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
         "hop_fp64_clover_xpay": {"ms_per_launch": round(ms64, 5),
-                                 "achieved_GBps": round(alg64 / (ms64 * 1e-3) / 1e9, 1)},
+                                 "achieved_GBps": round(alg64 / (ms64 * 1e-3) / 1e9, 1),
+                                 "traffic": traffic64,
+                                 "hbm_GBps": None if traffic64 is None else round(traffic64 / (ms64 * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
