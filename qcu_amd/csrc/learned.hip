@@ -20,6 +20,7 @@
 // swapped hop directions, the weight gradient is a two-stage deterministic
 // reduction of g_o(x) (T_t a_i)(x)^dag over sites and colours, and Adam runs on
 // the device.  Training minimises |Mhat K b - b|^2 / |b|^2.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -680,7 +681,11 @@ struct Net : Precond {
   int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
   int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in);
   int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
+  unsigned long long id = 0;
+  unsigned long long uid() const override { return id; }
 };
+
+static std::atomic<unsigned long long> g_net_ids{0};
 
 static size_t ndiv(size_t a, size_t b) { return (a + b - 1) / b; }
 
@@ -690,6 +695,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_CHECK((l->prec_mask >> P32) & 1, QCU_ERR_STATE, "net: FP32 links not resident");
   lat = l;
   nl = nlayers;
+  id = ++g_net_ids;
   for (int k = 0; k <= nl; ++k) {
     QCU_CHECK(ch[k] >= 1 && ch[k] <= kNetMaxChan, QCU_ERR_ARG, "net: 1 to 8 channels per layer");
     chan[k] = ch[k];

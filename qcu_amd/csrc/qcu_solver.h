@@ -19,13 +19,16 @@ struct Precond {
   // out = K in; must only enqueue work on S.stream (it is graph-captured);
   // st lets kernels skip systems that have stopped.
   virtual int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) = 0;
+  // unique over the process lifetime: a captured graph is valid only for the
+  // preconditioner (and device buffers) it was captured with
+  virtual unsigned long long uid() const = 0;
 };
 
 struct VecSet {
   int prec = P64, px = P64;
   Spinor r, rt, p, v, s, t, tmp, x, ph, sh;
   std::map<long, hipGraphExec_t> graphs;
-  Precond *graph_net = nullptr;
+  unsigned long long graph_net = 0;  // uid of the preconditioner the graphs were captured with
   int create(int prec, int px, int n, bool precond);
   ~VecSet();
 };

@@ -165,13 +165,20 @@ static int run_chunk(Solver &S, VecSet &V, Precond *K, int chunk, bool use_graph
     for (int k = 0; k < chunk; ++k) QCU_TRY(enqueue_iteration(S, V, K));
     return 0;
   }
+  if (K && V.graph_net != K->uid()) {
+    // graphs captured with another preconditioner hold its device buffers: drop them all
+    for (auto g = V.graphs.begin(); g != V.graphs.end();) {
+      if (g->first & 1) {
+        (void)hipGraphExecDestroy(g->second);
+        g = V.graphs.erase(g);
+      } else {
+        ++g;
+      }
+    }
+    V.graph_net = K->uid();
+  }
   const long key = ((long)chunk << 8) | (K ? 1 : 0);
   auto it = V.graphs.find(key);
-  if (it != V.graphs.end() && V.graph_net != K) {
-    (void)hipGraphExecDestroy(it->second);
-    V.graphs.erase(it);
-    it = V.graphs.end();
-  }
   if (it == V.graphs.end()) {
     hipGraph_t graph;
     QCU_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
@@ -184,7 +191,6 @@ static int run_chunk(Solver &S, VecSet &V, Precond *K, int chunk, bool use_graph
     QCU_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
     V.graphs[key] = exec;
-    V.graph_net = K;
     it = V.graphs.find(key);
   }
   QCU_HIP(hipGraphLaunch(it->second, S.stream));

@@ -157,3 +157,23 @@ def test_vector_layer_kernel_matches_oracle(orc, monkeypatch):
     e = rel(host(net.forward(dev(x))).ravel(), ref)
     log(f"vector layer kernel: forward {e:.1e}")
     assert e < 2e-5
+
+
+def test_replacing_the_network_recaptures_graphs(orc):
+    """a solve with a new network (after the previous one was destroyed) must not
+    replay graphs captured with the old one: graph and eager runs agree bit for bit"""
+    import torch
+    from qcu_amd.lattice import FP32, Lattice, LearnedPreconditioner
+    L = (4, 4, 4, 8)
+    U = orc.gauge_random(L, 0.3, 160)
+    lat = Lattice(L, 0.124, 1.0, -1)
+    lat.load_gauge(dev(U))
+    b = dev(orc.gauss((2, orc.vol(L) // 2, 4, 3), 161))
+    net1 = LearnedPreconditioner(lat, (1, 2, 1), seed=1, init_scale=0.3)
+    x1, r1 = lat.solve(b, tol=1e-10, prec=FP32, precond=net1, use_graph=True)
+    net1.close()
+    net2 = LearnedPreconditioner(lat, (1, 2, 1), seed=2, init_scale=0.3)
+    xg, rg = lat.solve(b, tol=1e-10, prec=FP32, precond=net2, use_graph=True)
+    xe, re_ = lat.solve(b, tol=1e-10, prec=FP32, precond=net2, use_graph=False)
+    log(f"net1 {r1.iters} iters; net2 graph {rg.iters} / eager {re_.iters} iters")
+    assert rg.converged and torch.equal(xg, xe) and rg.iters == re_.iters
