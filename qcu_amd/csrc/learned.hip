@@ -462,6 +462,153 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
   }
 }
 
+// Weight gradient on the matrix cores.  For term t,
+//   gW[t][o][i][s][s2] = sum over sites x of both parities and colours c of
+//                        g_o(x)[s][c] conj(v_i(x)[s2][c]),  v_i = T_t a_i,
+// i.e. D[m = (o, s)][n = (i, s2)] = sum_k G[m][k] conj(V)[k][n] over k = (x, c),
+// as four real FP32 products per colour on v_mfma_f32_16x16x4_f32:
+//   Dr += Gr Vr + Gi Vi,  Di += Gi Vr,  Dn += Gr Vi,  Im D = Di - Dn.
+// A K step is 4 sites of one colour: lane l (group q = l / 16, index n = l % 16)
+// feeds A[m = n][k = q] = g of row (o = 4 mt + n / 4, s = n % 4) at site q of
+// the group, and B[k = q][n] = v of column (i = 4 nt + n / 4, s2 = n % 4) at the
+// same site; it ends holding D rows (o = 4 mt + q, s = j), column n.  Each wave
+// sweeps groups of 4 sites; the 4 waves of a block are summed through LDS and
+// the block sums are added in a fixed order by k_net_wgrad_fin.
+constexpr int kWgBlocks = 256;  // blocks per term
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
+  const Geom g = a.g;
+  const int t = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  const int nsite = 2 * g.vh, ngroups = (nsite + 3) / 4;
+  const int nw = gridDim.x * 4;
+  const size_t vh = (size_t)g.vh;
+  const int rs = n & 3, rslot = (3 * rs) >> 1, rodd = (3 * rs) & 1;  // spin of this lane's row / column
+  f4v dr[MT][NT], di[MT][NT], dn[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      dr[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      di[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      dn[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
+    }
+  auto spin3 = [&](const float4 *b, cf (&v)[3]) {
+    const float4 x = b[0], y = b[vh];
+    if (rodd) {
+      v[0] = mk(x.z, x.w);
+      v[1] = mk(y.x, y.y);
+      v[2] = mk(y.z, y.w);
+    } else {
+      v[0] = mk(x.x, x.y);
+      v[1] = mk(x.z, x.w);
+      v[2] = mk(y.x, y.y);
+    }
+  };
+#pragma unroll 1
+  for (int grp = blockIdx.x * 4 + wave; grp < ngroups; grp += nw) {
+    int k = grp * 4 + q;
+    const bool ok = k < nsite;
+    if (!ok) k = grp * 4;  // a valid site; its contribution is zeroed below
+    const int p = k >= g.vh ? 1 : 0;
+    const int i = k - p * g.vh;
+    const Site site = decode(g, i, p);
+    const float keep = ok ? 1.f : 0.f;
+    // rows: output gradient of channels 4 mt + n / 4, spin n % 4
+    cf gv[MT][3];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int o = 4 * mt + (n >> 2);
+      if (o < a.cout) {
+        const SpinorPtr f = chan_field(a.gout, o, p, g.vh);
+        spin3((const float4 *)f.p + i + (size_t)rslot * vh, gv[mt]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gv[mt][c] = mk(keep * gv[mt][c].re, keep * gv[mt][c].im);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gv[mt][c] = mk(0.f, 0.f);
+      }
+    }
+    // columns: transported inputs of channels 4 nt + n / 4, spin n % 4
+    const TermGeo tg = term_geo(g, site, i, p, t);
+    cf u[9];
+    if (tg.hop) {
+      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) u[c] = mk(tg.ph * u[c].re, tg.ph * u[c].im);
+    }
+    cf vv[NT][3];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int ch = 4 * nt + (n >> 2);
+      if (ch < a.cin) {
+        const SpinorPtr f = chan_field(a.act, ch, tg.jp, g.vh);
+        spin3((const float4 *)f.p + tg.j + (size_t)rslot * vh, vv[nt]);
+        if (tg.hop) transport1(u, vv[nt], tg.fwd, 1.f);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vv[nt][c] = mk(0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const float ar = gv[mt][c].re, ai = gv[mt][c].im, br = vv[nt][c].re, bi = vv[nt][c].im;
+          dr[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, dr[mt][nt], 0, 0, 0);
+          dr[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dr[mt][nt], 0, 0, 0);
+          di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, di[mt][nt], 0, 0, 0);
+          dn[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, dn[mt][nt], 0, 0, 0);
+        }
+  }
+  // block sum in a fixed order: registers (mt, nt, part, j) of every lane
+  constexpr int NREG = MT * NT * 2 * 4;
+  __shared__ float sh[4][NREG][64];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = ((mt * NT + nt) * 2) * 4 + j;
+        sh[wave][r][lane] = dr[mt][nt][j];
+        sh[wave][r + 4][lane] = di[mt][nt][j] - dn[mt][nt][j];
+      }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NREG * 64; e += blockDim.x) {
+    const int r = e >> 6, l = e & 63;
+    const float v = (sh[0][r][l] + sh[1][r][l]) + (sh[2][r][l] + sh[3][r][l]);
+    part[((size_t)(t * gridDim.x + blockIdx.x) * NREG + r) * 64 + l] = v;
+  }
+}
+
+// gW[t][o][i][s][s2] = sum over the blocks of term t, FP64 accumulation in block order
+template <int MT, int NT>
+__global__ void k_net_wgrad_fin(const float *part, int nblk, int cin, int cout, float2 *gW) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 9 * cout * cin * 16
+  const int total = 9 * cout * cin * 16;
+  if (idx >= total) return;
+  const int e = idx % 16, s = e >> 2, s2 = e & 3;
+  const int ci = (idx / 16) % cin;
+  const int o = (idx / 16 / cin) % cout;
+  const int t = idx / 16 / cin / cout;
+  constexpr int NREG = MT * NT * 2 * 4;
+  const int mt = o >> 2, nt = ci >> 2;
+  const int lane = (o & 3) * 16 + (ci & 3) * 4 + s2;
+  const int r = ((mt * NT + nt) * 2) * 4 + s;
+  double re = 0, im = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const float *src = part + ((size_t)(t * nblk + b) * NREG + r) * 64 + lane;
+    re += src[0];
+    im += src[4 * 64];
+  }
+  gW[idx] = make_float2((float)re, (float)im);
+}
+
 // second stage: gW[t][o][ci][16] = sum over blocks (FP64 accumulation)
 __global__ void k_net_wgrad_sum(const float *partials, int nblk, int cin, int cout, int co, int ngroups,
                                 float2 *gW, int accumulate) {
@@ -728,6 +875,8 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
     const int co = chan[k + 1] >= 2 ? 2 : 1;
     wp = std::max(wp, (size_t)9 * chan[k] * ndiv(chan[k + 1], co) * kWgradBlocks * co * 32);
   }
+  // matrix-core weight gradient: 9 terms x blocks x (up to 2 x 2 tiles x 2 parts x 4) registers x 64 lanes
+  wp = std::max(wp, (size_t)9 * kWgBlocks * 32 * 64);
   QCU_TRY(wpart.alloc(wp * 4));
   // initialisation: the identity through channel 0 plus small seeded noise
   std::vector<float> h(nw * 2, 0.f);
@@ -845,16 +994,33 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
     a.cin = cin;
     a.cout = cout;
     a.partials = (float *)wpart.p;
-    const int co = cout >= 2 ? 2 : 1;
-    const int ng = (int)ndiv(cout, co);
-    dim3 grid(kWgradBlocks, 9 * cin, ng);
-    if (co == 2) hipLaunchKernelGGL(k_net_wgrad<2>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_net_wgrad<1>, grid, dim3(256), 0, s, a);
-    QCU_HIP(hipGetLastError());
     const int total = 9 * cout * cin * 16;
-    hipLaunchKernelGGL(k_net_wgrad_sum, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, (const float *)wpart.p,
-                       kWgradBlocks, cin, cout, co, ng, (float2 *)gW.p + woff[k], 0);
-    QCU_HIP(hipGetLastError());
+    if (use_mfma) {
+      const int mt = (cout + 3) / 4, nt = (cin + 3) / 4;
+      const dim3 grid(kWgBlocks, 9);
+      float *pp = (float *)wpart.p;
+      float2 *dst = (float2 *)gW.p + woff[k];
+      const dim3 fg((unsigned)ndiv(total, 256));
+#define QCU_WG_CASE(MT_, NT_)                                                                     \
+  if (mt == MT_ && nt == NT_) {                                                                   \
+    hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_>), grid, dim3(256), 0, s, a, pp);               \
+    hipLaunchKernelGGL((k_net_wgrad_fin<MT_, NT_>), fg, dim3(256), 0, s, (const float *)pp, kWgBlocks, cin, \
+                       cout, dst);                                                                \
+  }
+      QCU_WG_CASE(1, 1) QCU_WG_CASE(1, 2) QCU_WG_CASE(2, 1) QCU_WG_CASE(2, 2)
+#undef QCU_WG_CASE
+      QCU_HIP(hipGetLastError());
+    } else {
+      const int co = cout >= 2 ? 2 : 1;
+      const int ng = (int)ndiv(cout, co);
+      dim3 grid(kWgradBlocks, 9 * cin, ng);
+      if (co == 2) hipLaunchKernelGGL(k_net_wgrad<2>, grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL(k_net_wgrad<1>, grid, dim3(256), 0, s, a);
+      QCU_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_net_wgrad_sum, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s,
+                         (const float *)wpart.p, kWgradBlocks, cin, cout, co, ng, (float2 *)gW.p + woff[k], 0);
+      QCU_HIP(hipGetLastError());
+    }
     if (k == 0 && !want_grad_in) break;
     // data gradient: the forward layer with the transposed weights (cin <-> cout)
     SpinorPtr gnext;
