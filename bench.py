@@ -64,11 +64,21 @@ def cpu_baseline(U_host, b_host, iters_gpu_plain, sample_iters, threads):
     _, info = oracle.bicgstab(b_host, U_host, L, KAPPA, CSW, TB, tol=1e-30, max_iter=sample_iters)
     per_iter = info["iter_s"] / max(info["iters"], 1)
     est = info["setup_s"] + per_iter * iters_gpu_plain
+    # the host Wilson hop alone (one parity), median of 3
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.hop(b_host[1], U_host, L, 0, TB)
+        ts.append(time.perf_counter() - t0)
+    hop_s = sorted(ts)[1]
+    vh = b_host.shape[1]
     return {"value": round(est, 3), "unit": "s (FP64 time-to-solution, extrapolated)",
             "cores": nthreads, "kind": "port",
             "sample": f"{info['iters']} BiCGSTAB iterations of the same 32^3x64 clover system "
                       f"({info['iter_s']:.1f} s) + clover set-up ({info['setup_s']:.1f} s), "
-                      f"extrapolated to {iters_gpu_plain} iterations"}
+                      f"extrapolated to {iters_gpu_plain} iterations",
+            "hop_fp64": {"ms": round(hop_s * 1e3, 2), "GBps": round(HOP_BYTES[0] * vh / hop_s / 1e9, 2),
+                         "GFLOPs": round(1320 * vh / hop_s / 1e9, 2)}}
 
 
 def main():
@@ -160,6 +170,7 @@ def main():
     traffic64 = traffic_of(HOP64_TRAFFIC)
     ms64 = lat.bench_hop(prec=FP64, clover=True, xpay=True, iters=50)
     alg64 = (HOP_BYTES[0] + CLOVER_BYTES[0] + XPAY_BYTES[0]) * vh
+    ms64_plain = lat.bench_hop(prec=FP64, clover=False, xpay=False, iters=50)
 
     cpu = None
     if not args.no_cpu:
@@ -208,6 +219,9 @@ def main():
                          "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,
                          "hbm_GBps": None if traffic is None else round(traffic / (ms_hop * 1e-3) / 1e9, 1),
                          "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
+        "hop_fp64_wilson": {"ms_per_launch": round(ms64_plain, 5),
+                            "achieved_GBps": round(HOP_BYTES[0] * vh / (ms64_plain * 1e-3) / 1e9, 1),
+                            "GFLOPs": round(1320 * vh / (ms64_plain * 1e-3) / 1e9, 1)},
         "hop_fp64_clover_xpay": {"ms_per_launch": round(ms64, 5),
                                  "achieved_GBps": round(alg64 / (ms64 * 1e-3) / 1e9, 1),
                                  "traffic": traffic64,

@@ -55,3 +55,29 @@ def test_small_clover_solve_matches_fixture(orc):
     e = worst(host(x), 2, g["x_sites"])
     log(f"4^4 clover solve: {r.iters} iters (oracle {g['iters']}), solution sites {e:.1e}")
     assert r.converged and e < 1e-10
+
+
+def test_weak_field_configuration(orc):
+    """the reference's weak-field ILDG configuration (4^3x8): FP64 Wilson and
+    clover hops against the oracle, and a clover solve to 1e-10"""
+    from qcu_amd.gauge_io import read_ildg
+    from qcu_amd.lattice import FP16, FP64, Lattice
+    U, L = read_ildg(os.path.join(HERE, "golden", "weak_field.lime"))
+    L = tuple(L)
+    U = np.ascontiguousarray(U.reshape(4, 2, -1, 3, 3))
+    kappa, csw = 0.124, 1.0
+    lat = Lattice(L, kappa, csw, -1)
+    lat.load_gauge(dev(U))
+    psi = orc.gauss((orc.vol(L) // 2, 4, 3), 3)
+    for p in (0, 1):
+        h = host(lat.dslash(dev(psi), p, clover=False, prec=FP64))
+        c = host(lat.dslash(dev(psi), p, clover=True, prec=FP64))
+        eh = np.linalg.norm(h - orc.hop(psi, U, L, p, -1)) / np.linalg.norm(h)
+        ec = np.linalg.norm(c - orc.clover_dslash(psi, U, L, p, kappa * csw, -1)) / np.linalg.norm(c)
+        assert eh < 1e-12 and ec < 1e-12, (p, eh, ec)
+    b = dev(orc.gauss((2, orc.vol(L) // 2, 4, 3), 4))
+    for prec in (FP64, FP16):
+        x, r = lat.solve(b, tol=1e-10, prec=prec)
+        tr = orc.true_residual(host(x), host(b), U, L, kappa, csw, -1)
+        log(f"weak field, prec {prec}: {r.iters} iters, oracle true residual {tr:.1e}")
+        assert r.converged and tr <= 1e-10
