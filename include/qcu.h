@@ -99,7 +99,14 @@ typedef struct QcuSolveParam_s {
   int use_graph;         /* capture the chunk as a hipGraph and replay it */
   void *precond;         /* QcuNetHandle or NULL */
   int legacy_stop;       /* 1: stop on Re<r,r> < tol absolute (reference rule) */
+  int method;            /* QCU_METHOD_*: 0 (a zeroed struct) is BiCGSTAB */
 } QcuSolveParam;
+
+enum {
+  QCU_METHOD_BICGSTAB = 0, /* even-odd BiCGSTAB (include/lattice_bistabcg.h:221-340) */
+  QCU_METHOD_CGNR = 1      /* CG on the normal equations of the odd Schur system, FP64
+                              (completes include/lattice_bistabcg.h:341-450, which lacks D^+) */
+};
 
 typedef struct QcuSolveInfo_s {
   int iters;

@@ -30,7 +30,7 @@ class QcuParamStruct(C.Structure):
 class SolveParam(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
                 ("reliable_delta", C.c_double), ("chunk", C.c_int), ("use_graph", C.c_int),
-                ("precond", C.c_void_p), ("legacy_stop", C.c_int)]
+                ("precond", C.c_void_p), ("legacy_stop", C.c_int), ("method", C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -58,6 +58,13 @@ int check_dev_ptr(const void *p, const char *what) {
 
 namespace {
 
+int check_method(const QcuSolveParam *p) {
+  QCU_CHECK(p->method == QCU_METHOD_BICGSTAB || p->method == QCU_METHOD_CGNR, QCU_ERR_ARG, "unknown solver method");
+  QCU_CHECK(p->method != QCU_METHOD_CGNR || (p->precision == QCU_PREC_FP64 && !p->precond), QCU_ERR_ARG,
+            "the normal-equation CG runs in FP64 without a preconditioner");
+  return 0;
+}
+
 int check_param(const QcuParam *param) {
   QCU_CHECK(param != nullptr, QCU_ERR_ARG, "param is null");
   for (int k = 0; k < 4; ++k) QCU_CHECK(param->lattice_size[k] > 0, QCU_ERR_DIMS, "lattice_size must be positive");
@@ -318,6 +325,7 @@ int qcuSolve(QcuLatticeHandle h, void *x, const void *b, const QcuSolveParam *p,
   QCU_CHECK(h && p, QCU_ERR_ARG, "null argument");
   QCU_CHECK(p->precision >= 0 && p->precision <= 2, QCU_ERR_ARG, "bad precision");
   QCU_CHECK(p->tol > 0, QCU_ERR_ARG, "tolerance must be positive");
+  QCU_TRY(check_method(p));
   QCU_TRY(check_dev_ptr(x, "x"));
   QCU_TRY(check_dev_ptr(b, "b"));
   Lattice &lat = h->lat;
@@ -388,6 +396,7 @@ int qcuBatchSolve(QcuBatchHandle h, void *const *x, const void *const *b, const 
   QCU_CHECK(p->precision >= 0 && p->precision <= 2, QCU_ERR_ARG, "bad precision");
   QCU_CHECK(p->tol > 0, QCU_ERR_ARG, "tolerance must be positive");
   QCU_CHECK(p->precond == nullptr, QCU_ERR_ARG, "batched solves take no preconditioner");
+  QCU_TRY(check_method(p));
   for (const Lattice *l : h->op.lats)
     QCU_CHECK((l->prec_mask >> p->precision) & 1, QCU_ERR_STATE, "links of the requested precision are not resident");
   for (int k = 0; k < h->op.nsys; ++k) {

@@ -352,6 +352,103 @@ __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, co
   block_partial_rhs<2>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
 }
 
+// ---------------------------------------------------------------- CG on the normal equations (FP64)
+// x += alpha p ; res -= alpha q ; |res|^2 per system
+__global__ void k_cg_xr(SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, const SegDesc *segs, int nseg,
+                        const SysState *st, double *partials) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.sd) return;
+  double acc[1] = {0};
+  if (c.active) {
+    const double al = st[c.sys].alpha[0];
+    cplx<double> xx[12], rr[12], pp[12], qq[12];
+    SpinorIO<P64>::load(view<P64>(x, c), c.t, xx);
+    SpinorIO<P64>::load(view<P64>(p, c), c.t, pp);
+    SpinorIO<P64>::load(view<P64>(r, c), c.t, rr);
+    SpinorIO<P64>::load(view<P64>(q, c), c.t, qq);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      xx[k] = xx[k] + al * pp[k];
+      rr[k] = rr[k] - al * qq[k];
+      acc[0] += norm2(rr[k]);
+    }
+    SpinorIO<P64>::store(view<P64>(x, c), c.t, xx);
+    SpinorIO<P64>::store(view<P64>(r, c), c.t, rr);
+  }
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
+}
+
+// p = z + beta p
+__global__ void k_cg_p(SpinorPtr p, SpinorPtr z, const SegDesc *segs, int nseg, const SysState *st) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.active) return;
+  const double be = st[c.sys].beta[0];
+  cplx<double> pp[12], zz[12];
+  SpinorIO<P64>::load(view<P64>(z, c), c.t, zz);
+  if (be != 0.0) {
+    SpinorIO<P64>::load(view<P64>(p, c), c.t, pp);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) zz[k] = zz[k] + be * pp[k];
+  }
+  SpinorIO<P64>::store(view<P64>(p, c), c.t, zz);
+}
+
+// out = gamma5 Ao^-1 in on the odd sites (gamma5 = diag(1, 1, -1, -1) in this basis)
+__global__ void k_g5_clinv(SpinorPtr out, SpinorPtr in, const SegDesc *segs, int nseg, const SysState *st) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.active) return;
+  cplx<double> v[12];
+  SpinorIO<P64>::load(view<P64>(in, c), c.t, v);
+  const CloverPtr C = c.sd->Cinv[P64][1];
+  if (C.p) {
+    const int site = c.t / c.nrhs;
+    double cb[36];
+    clover_load_block(C, site, 0, cb);
+    clover_apply_block(cb, v, 0);
+    clover_load_block(C, site, 1, cb);
+    clover_apply_block(cb, v, 1);
+  }
+#pragma unroll
+  for (int k = 6; k < 12; ++k) v[k] = mk(-v[k].re, -v[k].im);
+  SpinorIO<P64>::store(view<P64>(out, c), c.t, v);
+}
+
+// z = r - kappa^2 gamma5 y ; |z|^2 per system
+__global__ void k_cg_z(SpinorPtr z, SpinorPtr r, SpinorPtr y, const SegDesc *segs, int nseg, const SysState *st,
+                       double *partials) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.sd) return;
+  double acc[1] = {0};
+  if (c.active) {
+    const double k2 = c.sd->kappa * c.sd->kappa;
+    cplx<double> rr[12], yy[12];
+    SpinorIO<P64>::load(view<P64>(r, c), c.t, rr);
+    SpinorIO<P64>::load(view<P64>(y, c), c.t, yy);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const double sg = k < 6 ? k2 : -k2;
+      rr[k] = rr[k] - sg * yy[k];
+      acc[0] += norm2(rr[k]);
+    }
+    SpinorIO<P64>::store(view<P64>(z, c), c.t, rr);
+  }
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
+}
+
+// |q|^2 per system of the running systems
+__global__ void k_cg_norm(SpinorPtr q, const SegDesc *segs, int nseg, const SysState *st, double *partials) {
+  const TCtx c = tctx(segs, nseg, st);
+  if (!c.sd) return;
+  double acc[1] = {0};
+  if (c.active) {
+    cplx<double> v[12];
+    SpinorIO<P64>::load(view<P64>(q, c), c.t, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[0] += norm2(v[k]);
+  }
+  block_partial_rhs<1>(acc, partials, blockIdx.x, c.t0, c.nrhs, c.sd->pcap);
+}
+
 // ---------------------------------------------------------------- finishing kernel
 // One block per system: fixed-order sum of the system's block partials, then
 // the epilogue.
@@ -372,7 +469,10 @@ __global__ void __launch_bounds__(1024) k_finish(const double *partials, const S
     acc_b += n;
   }
   if (seg < 0) return;
-  if (st && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO || kind == FK_RELIABLE) && st[sys].done) return;
+  if (st && (kind == FK_ALPHA || kind == FK_OMEGA || kind == FK_RHO || kind == FK_RELIABLE || kind == FK_CG_ALPHA ||
+             kind == FK_CG_RHO) &&
+      st[sys].done)
+    return;
   // this system's partials are contiguous: slots r * pcap + blk0 .. + nb - 1;
   // up to 8 independent loads in flight per thread (the kernel is latency bound)
   const double *part = partials + ((size_t)r * segs[seg].pcap + blk0) * NR;
@@ -424,6 +524,9 @@ __global__ void __launch_bounds__(1024) k_finish(const double *partials, const S
     case FK_OMEGA: fin_omega(s, tot[0], tot[1], tot[2]); break;
     case FK_RHO: fin_rho(s, tot[0], tot[1], tot[2]); break;
     case FK_RELIABLE: fin_reliable(s, tot[0], tot[1], tot[2]); break;
+    case FK_CG_INIT: fin_cg_init(s, tot[0], s->tmp[0]); break;
+    case FK_CG_ALPHA: fin_cg_alpha(s, tot[0]); break;
+    case FK_CG_RHO: fin_cg_rho(s, tot[0]); break;
     default: break;
   }
 }
@@ -595,6 +698,43 @@ int cl_axpy(const BatchOp &op, SpinorPtr out, SpinorPtr x, double a, bool a_is_k
   hipLaunchKernelGGL(k_cl_axpy, dim3(nb), dim3(bs), 0, s, out, x, a, (int)a_is_kappa, y, which, parity, QCU_SEGS);
   QCU_HIP(hipGetLastError());
   return 0;
+}
+
+int cg_xr(const BatchOp &op, SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, SysState *st, double *partials,
+          hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(bs), 0, s, x, r, p, q, QCU_SEGS, st, partials);
+  QCU_HIP(hipGetLastError());
+  return finish(op, partials, bs, 1, FK_STORE_TMP, st, nullptr, s);
+}
+
+int cg_p(const BatchOp &op, SpinorPtr p, SpinorPtr z, SysState *st, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(bs), 0, s, p, z, QCU_SEGS, st);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int g5_clinv(const BatchOp &op, SpinorPtr out, SpinorPtr in, SysState *st, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_g5_clinv, dim3(nb), dim3(bs), 0, s, out, in, QCU_SEGS, st);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int cg_z(const BatchOp &op, SpinorPtr z, SpinorPtr r, SpinorPtr y, int kind, SysState *st, double *partials,
+         hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_cg_z, dim3(nb), dim3(bs), 0, s, z, r, y, QCU_SEGS, st, partials);
+  QCU_HIP(hipGetLastError());
+  return finish(op, partials, bs, 1, kind, st, nullptr, s);
+}
+
+int cg_qnorm(const BatchOp &op, SpinorPtr q, SysState *st, double *partials, hipStream_t s) {
+  const int bs = 256, nb = batch_blocks(op, bs);
+  hipLaunchKernelGGL(k_cg_norm, dim3(nb), dim3(bs), 0, s, q, QCU_SEGS, st, partials);
+  QCU_HIP(hipGetLastError());
+  return finish(op, partials, bs, 1, FK_CG_ALPHA, st, nullptr, s);
 }
 
 int residual_parity(const BatchOp &op, SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, double *partials,

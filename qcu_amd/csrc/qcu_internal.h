@@ -182,4 +182,13 @@ int residual_parity(const BatchOp &op, SpinorPtr b, SpinorPtr x, SpinorPtr t, in
 int gen_gauge(void *ext, const Geom &g, double eps, uint64_t seed, hipStream_t s);
 int gen_gauss(void *ext, long n_complex, uint64_t seed, hipStream_t s);
 
+// CG on the normal equations (FP64, per-system device scalars)
+int cg_xr(const BatchOp &op, SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, SysState *st, double *partials,
+          hipStream_t s);
+int cg_p(const BatchOp &op, SpinorPtr p, SpinorPtr z, SysState *st, hipStream_t s);
+int g5_clinv(const BatchOp &op, SpinorPtr out, SpinorPtr in, SysState *st, hipStream_t s);
+int cg_z(const BatchOp &op, SpinorPtr z, SpinorPtr r, SpinorPtr y, int kind, SysState *st, double *partials,
+         hipStream_t s);
+int cg_qnorm(const BatchOp &op, SpinorPtr q, SysState *st, double *partials, hipStream_t s);
+
 }  // namespace qcu

@@ -54,6 +54,9 @@ enum FinKindAll {
   FK_RHO = 7,       // rho, |r|^2, beta
   FK_RELIABLE = 8,  // after a reliable update
   FK_STORE_TMP = 9, // st->tmp[k] = tot[k]
+  FK_CG_INIT = 10,  // CG on the normal equations: rho = |z|^2, beta = 0
+  FK_CG_ALPHA = 11, // alpha = rho / |q|^2
+  FK_CG_RHO = 12,   // rho' = |z|^2, beta = rho'/rho, rr = tmp[0] (|res|^2), stop test
 };
 
 }  // namespace qcu

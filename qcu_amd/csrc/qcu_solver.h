@@ -46,6 +46,7 @@ struct Solver {
   double *dscal = nullptr;   // 4 * nsys device scalars
   SysState *host_st = nullptr;
   double *host_scal = nullptr;
+  Spinor cg_z, cg_p, cg_q;  // CG on the normal equations (FP64), made on first use
   int init(const BatchOp *op, hipStream_t s);
   VecSet *get_set(int prec, bool precond, int *rc);
   ~Solver();

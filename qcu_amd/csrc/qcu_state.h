@@ -72,4 +72,42 @@ QCU_D void fin_rho(SysState *st, double rre, double rim, double rr) {
   if (st->delta2 > 0 && rr < st->delta2 * st->rr_ref) st->done = 3;
 }
 
+// ---- CG on the normal equations Mhat^+ Mhat x = Mhat^+ b (CGNR): real scalars
+// kept in the real parts of the state fields.  rr is the squared residual of
+// the Schur system, |b - Mhat x|^2, carried by the recursion res -= alpha q.
+QCU_D void fin_cg_init(SysState *st, double zz, double rr) {
+  st->rho[0] = zz;
+  st->beta[0] = 0;
+  st->rr = rr;
+  st->done = (rr < st->rr_stop) ? 1 : 0;
+}
+
+QCU_D void fin_cg_alpha(SysState *st, double qq) {
+  if (!(qq > 0) || !isfinite(qq)) {
+    st->done = 2;
+    return;
+  }
+  st->alpha[0] = st->rho[0] / qq;
+}
+
+QCU_D void fin_cg_rho(SysState *st, double zz) {
+  const double rr = st->tmp[0];
+  st->iter += 1;
+  st->rr = rr;
+  if (!isfinite(rr) || !isfinite(zz)) {
+    st->done = 2;
+    return;
+  }
+  if (rr < st->rr_stop) {
+    st->done = 1;
+    return;
+  }
+  if (!(st->rho[0] > 0)) {
+    st->done = 2;
+    return;
+  }
+  st->beta[0] = zz / st->rho[0];
+  st->rho[0] = zz;
+}
+
 }  // namespace qcu

@@ -316,6 +316,82 @@ static int schur_solve(Solver &S, int prec, const std::vector<double> &target2, 
   return 0;
 }
 
+// ---------------------------------------------------------------- CG on the normal equations
+// CGNR on the odd Schur system: CG on Mhat^+ Mhat x_o = Mhat^+ bhat in FP64, with
+//   Mhat^+ = 1 - kappa^2 gamma5 D_oe Ae^-1 D_eo gamma5 Ao^-1
+// (gamma5-hermiticity of the hop, A Hermitian and commuting with gamma5).  The
+// stop test uses the Schur residual |bhat - Mhat x_o|^2 carried by the
+// recursion res -= alpha Mhat p.  Scalars stay on the device (per system).
+static int cgnr_iteration(Solver &S) {
+  const BatchOp &op = *S.op;
+  const LaunchCfg cfg = tuned_cfg(op, P64);
+  SysState *st = S.st;
+  SpinorPtr none{};
+  const SpinorPtr z = S.cg_z.ptr(), p = S.cg_p.ptr(), q = S.cg_q.ptr(), t1 = S.t64.ptr(), t2 = S.xe.ptr();
+  // q = Mhat p ; alpha = rho / |q|^2
+  QCU_TRY(hop(op, P64, t1, p, 0, EPI_PLAIN, none, RED_NONE, none, st, nullptr, cfg, S.stream));
+  QCU_TRY(hop(op, P64, q, t1, 1, EPI_XPAY, p, RED_NONE, none, st, nullptr, cfg, S.stream));
+  QCU_TRY(cg_qnorm(op, q, st, S.partials, S.stream));
+  // x += alpha p ; res -= alpha q ; |res|^2
+  QCU_TRY(cg_xr(op, S.xo.ptr(), S.r64.ptr(), p, q, st, S.partials, S.stream));
+  // z = Mhat^+ res ; beta = |z|^2 / rho ; p = z + beta p
+  QCU_TRY(g5_clinv(op, t1, S.r64.ptr(), st, S.stream));
+  QCU_TRY(hop(op, P64, t2, t1, 0, EPI_PLAIN, none, RED_NONE, none, st, nullptr, cfg, S.stream, true));
+  QCU_TRY(hop(op, P64, t1, t2, 1, EPI_PLAIN, none, RED_NONE, none, st, nullptr, cfg, S.stream, false));
+  QCU_TRY(cg_z(op, z, S.r64.ptr(), t1, FK_CG_RHO, st, S.partials, S.stream));
+  QCU_TRY(cg_p(op, p, z, st, S.stream));
+  return 0;
+}
+
+static int cgnr_solve(Solver &S, const std::vector<double> &target2, int max_iter, int chunk, bool warm,
+                      const std::vector<int> &active, std::vector<SolveStats> &stats) {
+  const BatchOp &op = *S.op;
+  const int nsys = S.nsys;
+  if (!S.cg_z.data.p) {
+    QCU_TRY(S.cg_z.create(P64, S.n));
+    QCU_TRY(S.cg_p.create(P64, S.n));
+    QCU_TRY(S.cg_q.create(P64, S.n));
+  }
+  SysState *hs = S.host_st;
+  std::memset(hs, 0, nsys * sizeof(SysState));
+  for (int k = 0; k < nsys; ++k) {
+    hs[k].rr_stop = target2[k];
+    hs[k].done = active[k] ? 0 : 4;
+  }
+  QCU_TRY(write_states(S));
+  const LaunchCfg cfg = tuned_cfg(op, P64);
+  SpinorPtr none{};
+  if (!warm) QCU_TRY(zero_spinor(op, S.xo.ptr(), P64, S.stream));
+  // res = bhat - Mhat x_o, |res|^2 into tmp[0]
+  QCU_TRY(apply_mhat64(S, S.r64.ptr(), S.xo.ptr(), S.t64.ptr()));
+  QCU_TRY(xmy_norm(op, S.bh.ptr(), S.r64.ptr(), S.st, S.partials, 1, S.stream));
+  // z = Mhat^+ res, rho = |z|^2, p = z
+  QCU_TRY(g5_clinv(op, S.t64.ptr(), S.r64.ptr(), S.st, S.stream));
+  QCU_TRY(hop(op, P64, S.xe.ptr(), S.t64.ptr(), 0, EPI_PLAIN, none, RED_NONE, none, S.st, nullptr, cfg, S.stream,
+              true));
+  QCU_TRY(hop(op, P64, S.t64.ptr(), S.xe.ptr(), 1, EPI_PLAIN, none, RED_NONE, none, S.st, nullptr, cfg, S.stream,
+              false));
+  QCU_TRY(cg_z(op, S.cg_z.ptr(), S.r64.ptr(), S.t64.ptr(), FK_CG_INIT, S.st, S.partials, S.stream));
+  QCU_TRY(cg_p(op, S.cg_p.ptr(), S.cg_z.ptr(), S.st, S.stream));
+  QCU_TRY(read_states(S));
+  for (int k = 0; k < nsys; ++k) stats[k] = SolveStats{0, 0, 0, 0};
+  int iters = 0;
+  for (;;) {
+    bool running = false;
+    for (int k = 0; k < nsys; ++k) running = running || hs[k].done == 0;
+    if (!running || iters >= max_iter) break;
+    const int n = std::min(chunk, max_iter - iters);
+    for (int j = 0; j < n; ++j) QCU_TRY(cgnr_iteration(S));
+    QCU_TRY(read_states(S));
+    for (int k = 0; k < nsys; ++k) iters = std::max(iters, hs[k].iter);
+  }
+  for (int k = 0; k < nsys; ++k) {
+    stats[k].iters = hs[k].iter;
+    stats[k].converged = hs[k].done == 1 ? 1 : 0;
+  }
+  return 0;
+}
+
 // per-system |b - M x|^2 and |b|^2 of the internal FP64 fields into S.host_scal[2k], [2k+1]
 static int residuals_internal(Solver &S, SpinorPtr xe, SpinorPtr xo, SpinorPtr be, SpinorPtr bo, SpinorPtr t) {
   const BatchOp &op = *S.op;
@@ -415,8 +491,11 @@ int full_solve(Solver &S, void *const *x_ext, const void *const *b_ext, const Qc
     int budget = max_iter;
     for (int k = 0; k < nsys; ++k)
       if (active[k]) budget = std::min(budget, max_iter - total_iters[k]);
-    QCU_TRY(schur_solve(S, prm.precision, target2, delta, std::max(budget, 0), chunk, prm.use_graph != 0, K, warm,
-                        active, st));
+    if (prm.method == QCU_METHOD_CGNR)
+      QCU_TRY(cgnr_solve(S, target2, std::max(budget, 0), chunk, warm, active, st));
+    else
+      QCU_TRY(schur_solve(S, prm.precision, target2, delta, std::max(budget, 0), chunk, prm.use_graph != 0, K,
+                          warm, active, st));
     trace.mark("schur solve");
     bool stalled = false;
     for (int k = 0; k < nsys; ++k) {

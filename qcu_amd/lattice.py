@@ -17,6 +17,7 @@ from ._lib import SolveInfo, SolveParam
 
 FP64, FP32, FP16 = 0, 1, 2
 PREC = {"fp64": FP64, "fp32": FP32, "fp16": FP16}
+METHODS = {"bicgstab": 0, "cgnr": 1}
 
 
 def _ptr(t: torch.Tensor) -> C.c_void_p:
@@ -124,13 +125,15 @@ class Lattice:
         return r.value
 
     def solve(self, b, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
-              use_graph=True, precond=None, legacy_stop=False, x=None):
+              use_graph=True, precond=None, legacy_stop=False, x=None, method="bicgstab"):
+        """x with |b - M x| / |b| <= tol; method "bicgstab" (any precision,
+        optional preconditioner) or "cgnr" (CG on the normal equations, FP64)"""
         if x is None:
             x = torch.empty_like(b)
         p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec),
                        reliable_delta=reliable_delta, chunk=chunk, use_graph=int(use_graph),
                        precond=None if precond is None else precond.handle,
-                       legacy_stop=int(legacy_stop))
+                       legacy_stop=int(legacy_stop), method=METHODS[method])
         info = SolveInfo()
         _lib.check(self._lib.qcuSolve(self.h, _ptr(x), _ptr(b), C.byref(p), C.byref(info)))
         return x, SolveResult(info.iters, info.reliable_updates, bool(info.converged),
@@ -187,14 +190,15 @@ class Batch:
             pass
 
     def solve(self, bs, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
-              use_graph=True, xs=None):
+              use_graph=True, xs=None, method="bicgstab"):
         assert len(bs) == self.nsys
         if xs is None:
             xs = [torch.empty_like(b) for b in bs]
         bp = (C.c_void_p * self.nsys)(*[_ptr(b).value for b in bs])
         xp = (C.c_void_p * self.nsys)(*[_ptr(x).value for x in xs])
         p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec), reliable_delta=reliable_delta,
-                       chunk=chunk, use_graph=int(use_graph), precond=None, legacy_stop=0)
+                       chunk=chunk, use_graph=int(use_graph), precond=None, legacy_stop=0,
+                       method=METHODS[method])
         info = (SolveInfo * self.nsys)()
         _lib.check(self._lib.qcuBatchSolve(self.h, xp, bp, C.byref(p), info))
         res = [SolveResult(i.iters, i.reliable_updates, bool(i.converged), bool(i.fallback), i.true_res,

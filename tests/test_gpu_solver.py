@@ -108,3 +108,45 @@ def test_nan_source_reports_not_converged(orc):
     x, res = lat.solve(dev(b), tol=1e-10, prec=0, max_iter=50)
     log(f"nan source: converged={res.converged} iters={res.iters}")
     assert not res.converged
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_cg_normal_equations(orc, csw):
+    """CG on the normal equations of the Schur system (FP64): the oracle's true
+    residual meets 1e-10 and the solution is BiCGSTAB's; CG needs Mhat^+, so a
+    wrong adjoint would not converge"""
+    from qcu_amd.lattice import FP32, FP64, Lattice
+    from qcu_amd._lib import QcuError
+    L, kappa, tb = (8, 8, 4, 8), 0.122, -1
+    U = orc.gauge_random(L, 0.3, 180)
+    lat = Lattice(L, kappa, csw, tb)
+    lat.load_gauge(dev(U))
+    b = dev(orc.gauss((2, orc.vol(L) // 2, 4, 3), 181))
+    x, r = lat.solve(b, tol=1e-10, method="cgnr")
+    tr = orc.true_residual(host(x), host(b), U, L, kappa, csw, tb)
+    xb, rb = lat.solve(b, tol=1e-10, prec=FP64)
+    log(f"csw={csw}: CGNR {r.iters} iters (BiCGSTAB {rb.iters}), oracle true residual {tr:.1e}, "
+        f"|x - x_bicgstab|/|x| {rel(host(x), host(xb)):.1e}")
+    assert r.converged and tr <= 1e-10
+    assert rel(host(x), host(xb)) < 1e-8
+    with pytest.raises(QcuError):
+        lat.solve(b, tol=1e-10, prec=FP32, method="cgnr")
+
+
+def test_cg_normal_equations_batch(orc):
+    from qcu_amd.lattice import Batch, Lattice
+    shapes = [(4, 4, 4, 8), (6, 4, 4, 4)]
+    lats, Us = [], []
+    for k, L in enumerate(shapes):
+        U = orc.gauge_random(L, 0.3, 190 + k)
+        lat = Lattice(L, 0.124, 1.0, -1)
+        lat.load_gauge(dev(U))
+        lats.append(lat)
+        Us.append(U)
+    batch = Batch(lats, [2, 1])
+    bs = [dev(orc.gauss((2, orc.vol(shapes[0 if k < 2 else 1]) // 2, 4, 3), 195 + k)) for k in range(3)]
+    xs, res = batch.solve(bs, tol=1e-10, method="cgnr")
+    for k in range(3):
+        L, U = shapes[0 if k < 2 else 1], Us[0 if k < 2 else 1]
+        tr = orc.true_residual(host(xs[k]), host(bs[k]), U, L, 0.124, 1.0, -1)
+        assert res[k].converged and tr <= 1e-10, (k, tr)
