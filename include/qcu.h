@@ -57,6 +57,17 @@ void ncclBistabCgQcu(void *fermion_out, void *fermion_in, void *gauge, QcuParam 
                      QcuParam *grid);
 
 /* ---- errors (new) ------------------------------------------------------- */
+enum {
+  QCU_OK = 0,
+  QCU_ERR_ARG = 1,            /* null / host pointer, bad size or value */
+  QCU_ERR_DIMS = 2,           /* lattice extents */
+  QCU_ERR_GRID = 3,           /* grid != {1,1,1,1}: this build runs on one GPU */
+  QCU_ERR_NOT_CONVERGED = 4,
+  QCU_ERR_BREAKDOWN = 5,
+  QCU_ERR_ALLOC = 6,          /* device allocation failed */
+  QCU_ERR_STATE = 7,          /* gauge not loaded, precision not resident, ... */
+  QCU_ERR_HIP_BASE = 100      /* a HIP call failed: code = 100 + hipError_t */
+};
 int qcuGetLastError(void);
 const char *qcuGetLastErrorString(void);
 void qcuClearError(void);

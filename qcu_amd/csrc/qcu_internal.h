@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/qcu.h"
 #include "qcu_batch.h"
 #include "qcu_common.h"
 
@@ -22,8 +23,8 @@ int last_error_code();
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \
     if (e_ != hipSuccess) {                                                                \
-      ::qcu::set_error(100 + (int)e_, std::string(#call) + ": " + hipGetErrorString(e_));  \
-      return 100 + (int)e_;                                                                \
+      ::qcu::set_error(QCU_ERR_HIP_BASE + (int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return QCU_ERR_HIP_BASE + (int)e_;                                                   \
     }                                                                                      \
   } while (0)
 
@@ -41,16 +42,8 @@ int last_error_code();
     if (rc_) return rc_; \
   } while (0)
 
-enum ErrCode {
-  QCU_OK = 0,
-  QCU_ERR_ARG = 1,
-  QCU_ERR_DIMS = 2,
-  QCU_ERR_GRID = 3,
-  QCU_ERR_NOT_CONVERGED = 4,
-  QCU_ERR_BREAKDOWN = 5,
-  QCU_ERR_ALLOC = 6,
-  QCU_ERR_STATE = 7,
-};
+// error codes: the QCU_ERR_* enum of include/qcu.h; a failed HIP call returns
+// QCU_ERR_HIP_BASE + its hipError_t
 
 // --------------------------------------------------------------------------
 // device allocations
