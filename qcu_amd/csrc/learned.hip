@@ -187,30 +187,20 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
 }
 
 // ---- matrix-core layer -----------------------------------------------------
-// The channel and spin mixing of one layer is a complex GEMM per colour,
-//   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
-//   k = (term t, input channel i, spin s2),  n = site,
-// done as four real products on v_mfma_f32_16x16x4_f32 in FP32:
-//   Dr += Wr Vr,  Dn += Wi Vi,  Di += Wi Vr,  Di += Wr Vi,  Re D = Dr - Dn.
-// Lane l of an MFMA step holds A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
-// and returns D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.  A K step is the four
-// spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
-// so no transport is repeated.  An M tile is four output channels times four
-// spins, so lane group g ends with all 12 colour-spin components of channel
-// 4 mt + g at its site.  Each wave owns 16 sites.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// A operands of one layer: Apk[((t * cin + i) * MT + mt) * 2 + part][lane],
-// part 0 = Re W, 1 = Im W, row m = lane % 16 -> (o = 4 mt + m / 4, s = m % 4),
-// column k = lane / 16 -> s2
+// A operands of one layer: Apk[((t * cin + i) * MT + mt) * 3 + part][lane],
+// part 0 = Re W, 1 = Im W, 2 = -Im W, row m = lane % 16 -> (o = 4 mt + m / 4,
+// s = m % 4), column k = lane / 16 -> s2
+constexpr int kApkParts = 3;
 __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int mtiles) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int total = 9 * cin * mtiles * 2 * 64;
+  const int total = 9 * cin * mtiles * kApkParts * 64;
   if (idx >= total) return;
   const int lane = idx & 63;
   int r = idx >> 6;
-  const int part = r & 1;
-  r >>= 1;
+  const int part = r % kApkParts;
+  r /= kApkParts;
   const int mt = r % mtiles;
   r /= mtiles;
   const int i = r % cin, t = r / cin;
@@ -219,31 +209,82 @@ __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int m
   float v = 0.f;
   if (o < cout) {
     const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + sp * 4 + s2];
-    v = part ? w.y : w.x;
+    v = part == 0 ? w.x : part == 1 ? w.y : -w.y;
   }
   Apk[idx] = v;
 }
 
-QCU_D void transport1(const cf (&u)[9], cf (&v)[3], int fwd, float ph) {
+// the colour matrix of a term, applied as v <- M v: M = ph U for a forward
+// hop and ph U^dag for a backward one, from the two stored rows of U; the
+// identity for the site term
+QCU_D void term_matrix(const float4 (&raw)[3], int hop, int fwd, float ph, cf (&m)[9]) {
+  if (!hop) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = mk(k % 4 == 0 ? 1.f : 0.f, 0.f);
+    return;
+  }
+  cf u[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u[2 * k] = mk(raw[k].x, raw[k].y);
+    u[2 * k + 1] = mk(raw[k].z, raw[k].w);
+  }
+  recon_row2(u);
+  if (fwd) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = mk(ph * u[k].re, ph * u[k].im);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[r * 3 + c] = mk(ph * u[c * 3 + r].re, -ph * u[c * 3 + r].im);
+  }
+}
+
+// v <- M v as a chain of fused multiply-adds
+QCU_D void transport1(const cf (&m)[9], cf (&v)[3]) {
   cf a[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    cf acc = mk(0.f, 0.f);
+    float re = m[r * 3].re * v[0].re, im = m[r * 3].re * v[0].im;
+    re = fmaf(-m[r * 3].im, v[0].im, re);
+    im = fmaf(m[r * 3].im, v[0].re, im);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (fwd) cmad(acc, u[r * 3 + c], v[c]);
-      else cmadj(acc, u[c * 3 + r], v[c]);
+    for (int c = 1; c < 3; ++c) {
+      re = fmaf(m[r * 3 + c].re, v[c].re, re);
+      re = fmaf(-m[r * 3 + c].im, v[c].im, re);
+      im = fmaf(m[r * 3 + c].re, v[c].im, im);
+      im = fmaf(m[r * 3 + c].im, v[c].re, im);
     }
-    a[r] = mk(ph * acc.re, ph * acc.im);
+    a[r] = mk(re, im);
   }
 #pragma unroll
   for (int r = 0; r < 3; ++r) v[r] = a[r];
 }
 
-// The term is the outer loop: the link of each term is loaded and
-// reconstructed once and the input channels stream through it.  The operands
-// of the next (term, channel) step -- spinor, weights, and at a term boundary
-// the next link -- are loaded one step ahead.
+// The channel and spin mixing of one layer is a complex GEMM per colour,
+//   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
+//   k = (term t, input channel i, spin s2),  n = site,
+// done as four real products on v_mfma_f32_16x16x4_f32 in FP32:
+//   Re D += Wr Vr + (-Wi) Vi,  Im D += Wi Vr + Wr Vi.
+// Lane l of an MFMA step holds A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
+// and returns D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.  A K step is the four
+// spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
+// so no transport is repeated.  An M tile is four output channels times four
+// spins, so lane group g ends with all 12 colour-spin components of channel
+// 4 mt + g at its site.  Each wave owns 16 sites.
+//
+// The term is the outer loop: the colour matrix of each term is built once and
+// the input channels stream through it; the link of the next term is loaded
+// while the current one runs.  The spinor and weight operands of each step are
+// loaded one step ahead into the other of two operand sets, which swap roles
+// every step; no load sits behind a branch (the last step loads a valid dummy),
+// so each step waits only for its own operands.
+struct MfmaOps {
+  float2 s[3];        // the lane's spin: colours 0..2
+  float w[2][3];      // [mt][Re W, Im W, -Im W]
+};
+
 template <int MT>
 QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   if (a.st && a.st->done) return;
@@ -258,16 +299,21 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   const bool ok = i < g.vh;
   if (!ok) i = base;
   const Site site = decode(g, i, p);
-  const int odd = (3 * gq) & 1, slot = (3 * gq) >> 1;
-  const size_t vh = (size_t)g.vh;
-  // Re D = Wr Vr - Wi Vi is kept as two sums (dr, dn) subtracted at the end
-  f4v dr[3][MT], dn[3][MT], di[3][MT];
+  // spin gq of a site is floats 6 gq .. 6 gq + 5 of its 24; each float2 of it
+  // lies in one float4 plane of the site-fastest layout
+  const unsigned vh16 = (unsigned)g.vh * 16u;
+  unsigned poff[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int f = 6 * gq + 2 * k;
+    poff[k] = (unsigned)(f >> 2) * vh16 + (unsigned)(f & 3) * 4u;
+  }
+  f4v dr[3][MT], di[3][MT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
-      dn[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
       di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
   const int cin = a.cin;
@@ -277,99 +323,94 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   if (a.zero_par == 1 - p) t_lo = 8;
   if (a.zero_par == p) t_hi = 8;
   const int q0 = t_lo * cin, nq = t_hi * cin;
-  // the lane's spin (two float4 slots) of channel ci at the neighbour of term geometry tg
-  auto spin_ptr = [&](const TermGeo &tg, int ci) {
-    const SpinorPtr in = chan_field(a.in, ci, tg.jp, g.vh);
-    return (const float4 *)in.p + tg.j + (size_t)slot * vh;
+  const size_t field = (size_t)g.vh * 96, chan_bytes = 2 * field;
+  // per term: the input field's parity base and the lane's three offsets
+  struct TermIn {
+    const char *fb;
+    unsigned o[3];
+  };
+  auto term_in = [&](const TermGeo &tg) {
+    TermIn r;
+    r.fb = (const char *)a.in.p + (size_t)tg.jp * field;
+    const unsigned jo = (unsigned)tg.j * 16u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.o[k] = jo + poff[k];
+    return r;
   };
   auto link_raw = [&](const TermGeo &tg, float4 (&l)[3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
   };
-  // operands of the first step
-  TermGeo ntg = term_geo(g, site, i, p, t_lo);
-  float4 nl[3];
-  if (ntg.hop) link_raw(ntg, nl);
-  const float4 *sp = spin_ptr(ntg, 0);
-  float4 na = sp[0], nb = sp[vh];
-  float nwr[MT], nwi[MT];
+  auto load_ops = [&](MfmaOps &x, const TermIn &ti, int ci, int q) {
+    const char *cb = ti.fb + (size_t)ci * chan_bytes;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    nwr[mt] = Apk[(size_t)(q0 * MT * 2 + mt * 2) * 64 + lane];
-    nwi[mt] = Apk[(size_t)(q0 * MT * 2 + mt * 2 + 1) * 64 + lane];
-  }
-  TermGeo tg = ntg;
-  cf u[9];
-  int t = t_lo, ci = 0;
-#pragma unroll 1
-  for (int q = q0; q < nq; ++q) {
-    if (ci == 0) {  // a new term: its link was loaded one step ahead
-      tg = ntg;
-      if (tg.hop) {
+    for (int k = 0; k < 3; ++k) x.s[k] = ld_at<float2>(cb, 0, ti.o[k]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          u[2 * k] = mk(nl[k].x, nl[k].y);
-          u[2 * k + 1] = mk(nl[k].z, nl[k].w);
-        }
-        recon_row2(u);
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int k = 0; k < 9; ++k) u[k] = mk(tg.ph * u[k].re, tg.ph * u[k].im);
-      }
-    }
-    const float4 ca = na, cb = nb;
-    float wr[MT], wi[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      wr[mt] = nwr[mt];
-      wi[mt] = nwi[mt];
-    }
-    // operands of step q + 1
-    if (q + 1 < nq) {
-      const float4 *np;
-      if (ci + 1 < cin) {
-        np = spin_ptr(tg, ci + 1);
-      } else {
-        ntg = term_geo(g, site, i, p, t + 1);
-        if (ntg.hop) link_raw(ntg, nl);
-        np = spin_ptr(ntg, 0);
-      }
-      na = np[0];
-      nb = np[vh];
-      const float *A = Apk + (size_t)((q + 1) * MT * 2) * 64 + lane;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        nwr[mt] = A[(mt * 2) * 64];
-        nwi[mt] = A[(mt * 2 + 1) * 64];
-      }
-    }
+      for (int part = 0; part < 3; ++part)
+        x.w[mt][part] = ld_at<float>(Apk, (((size_t)q * MT + mt) * 3 + part) * 256, (unsigned)lane * 4u);
+  };
+  auto compute = [&](const MfmaOps &x, const cf (&m)[9]) {
     cf v[3];
-    if (odd) {
-      v[0] = mk(ca.z, ca.w);
-      v[1] = mk(cb.x, cb.y);
-      v[2] = mk(cb.z, cb.w);
-    } else {
-      v[0] = mk(ca.x, ca.y);
-      v[1] = mk(ca.z, ca.w);
-      v[2] = mk(cb.x, cb.y);
-    }
-    if (tg.hop) transport1(u, v, tg.fwd, 1.f);
 #pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = mk(x.s[k].x, x.s[k].y);
+    transport1(m, v);  // the site term's matrix is the identity
     // lanes past the last site hold a copy of a valid site: their columns of D
     // are computed and never stored
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float br = v[c].re, bi = v[c].im;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], br, dr[c][mt], 0, 0, 0);
-        dn[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], bi, dn[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wi[mt], br, di[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wr[mt], bi, di[c][mt], 0, 0, 0);
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][0], br, dr[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][1], br, di[c][mt], 0, 0, 0);
+        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][2], bi, dr[c][mt], 0, 0, 0);
+        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][0], bi, di[c][mt], 0, 0, 0);
       }
     }
-    if (++ci == cin) {
-      ci = 0;
-      ++t;
+  };
+  // prologue: first term's link and first step's operands
+  TermGeo tg = term_geo(g, site, i, p, t_lo);
+  TermIn cur = term_in(tg);
+  float4 nl[3];
+  if (tg.hop) link_raw(tg, nl);
+  MfmaOps A, B;
+  load_ops(A, cur, 0, q0);
+#pragma unroll 1
+  for (int t = t_lo; t < t_hi; ++t) {
+    cf m[9];
+    term_matrix(nl, tg.hop, tg.fwd, tg.ph, m);
+    // the next term: geometry and link (a dummy copy of this term after the last)
+    TermGeo ng = tg;
+    TermIn nxt = cur;
+    if (t + 1 < t_hi) {
+      ng = term_geo(g, site, i, p, t + 1);
+      nxt = term_in(ng);
+      if (ng.hop) link_raw(ng, nl);
     }
+    const int qt = t * cin;
+    const int qn = qt + cin < nq ? qt + cin : nq - 1;  // first step of the next term
+    int ci = 0;
+#pragma unroll 1
+    for (; ci + 1 < cin; ci += 2) {
+      // the scheduling barriers keep each prefetch ahead of the step it overlaps
+      load_ops(B, cur, ci + 1, qt + ci + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(A, m);
+      const bool here = ci + 2 < cin;
+      load_ops(A, here ? cur : nxt, here ? ci + 2 : 0, here ? qt + ci + 2 : qn);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(B, m);
+    }
+    if (ci < cin) {  // odd channel count: the last channel of the term
+      load_ops(B, nxt, 0, qn);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(A, m);
+      A = B;
+    }
+    tg = ng;
+    cur = nxt;
   }
   if (!ok) return;
 #pragma unroll
@@ -380,7 +421,7 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j] - dn[c][mt][j], di[c][mt][j]);
+      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j], di[c][mt][j]);
     SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
   }
 }
@@ -535,9 +576,11 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     const TermGeo tg = term_geo(g, site, i, p, t);
     cf u[9];
     if (tg.hop) {
-      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
+      float4 raw[3];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) u[c] = mk(tg.ph * u[c].re, tg.ph * u[c].im);
+      for (int k = 0; k < 3; ++k)
+        raw[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
+      term_matrix(raw, 1, tg.fwd, tg.ph, u);
     }
     cf vv[NT][3];
 #pragma unroll
@@ -546,7 +589,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       if (ch < a.cin) {
         const SpinorPtr f = chan_field(a.act, ch, tg.jp, g.vh);
         spin3((const float4 *)f.p + tg.j + (size_t)rslot * vh, vv[nt]);
-        if (tg.hop) transport1(u, vv[nt], tg.fwd, 1.f);
+        if (tg.hop) transport1(u, vv[nt]);
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) vv[nt][c] = mk(0.f, 0.f);
@@ -901,7 +944,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_HIP(hipMemset(m.p, 0, nw * 8));
   QCU_HIP(hipMemset(v.p, 0, nw * 8));
   adam_t = 0;
-  QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 4 * 2 * 64 * 4));
+  QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 2 * kApkParts * 64 * 4));
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
   if (const char *e = getenv("QCU_NET_SHAPE")) {
@@ -932,7 +975,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   // transports cost the same, so that layer runs on the vector units
   if (use_mfma && cout > 1) {
     const int mtiles = (cout + 3) / 4;
-    const int total = 9 * cin * mtiles * 2 * 64;
+    const int total = 9 * cin * mtiles * kApkParts * 64;
     hipLaunchKernelGGL(k_net_pack, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, Wl, (float *)Apk.p, cin, cout,
                        mtiles);
     const int wsites = 16 * (block / 64);  // sites per block
