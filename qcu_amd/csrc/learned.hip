@@ -42,6 +42,24 @@ constexpr int kNetMaxLayers = 8;
 constexpr int kNetMaxChan = 8;
 constexpr int kWgradBlocks = 120;  // site chunks per weight-gradient combination
 
+// Order in which the waves of the matrix-core layer visit the lattice.  A wave
+// takes a chunk of 16 consecutive half-sites.  In the tiled order (an x row
+// is a whole number of chunks) the two output parities of a chunk are
+// neighbours, t runs fastest over the rows of a ty x tz cross-section tile,
+// and the tiles come last; with the XCD-aware block remap each XCD walks whole
+// tiles, so the t, y and z neighbours a wave reads are read by waves running
+// at about the same time on the same XCD and come from its L2.  The input
+// parity a site term reads is the one the hop terms of the other output
+// parity read, so running both parities together also shares those reads.
+struct WaveOrder {
+  int tiled;   // 0: chunks in site order, parity from the grid's y index
+  int nx;      // chunks per x row
+  int ty, tz;  // rows per tile in y and z
+  int nyt;     // tiles along y
+  int npar;    // output parities (1 or 2)
+  int nwaves;  // waves in the tiled grid
+};
+
 struct NetLayerArgs {
   SpinorPtr in;   // channel c, parity p at in.p + (c * 2 + p) * vh * 96 bytes, stride vh
   SpinorPtr out;  // same layout
@@ -53,6 +71,7 @@ struct NetLayerArgs {
   int zero_par;   // parity on which the input is known to be zero (-1: none)
   int ngroups;    // output-channel groups
   const SysState *st;
+  WaveOrder ord;  // matrix-core kernel only
 };
 
 QCU_D SpinorPtr chan_field(const SpinorPtr &base, int c, int p, int vh) {
@@ -285,15 +304,39 @@ struct MfmaOps {
   float w[2][3];      // [mt][Re W, Im W, -Im W]
 };
 
-template <int MT>
+template <int MT, int CI>
 QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int blk = remap_block(blockIdx.x, gridDim.x, 1);
-  const int base = (blk * (int)(blockDim.x >> 6) + wave) * 16;
-  const int p = a.parity0 + blockIdx.y;
-  if (base >= g.vh) return;  // uniform over the wave
+  const int w = blk * (int)(blockDim.x >> 6) + wave;
+  int base, p;
+  if (a.ord.tiled) {
+    if (w >= a.ord.nwaves) return;  // uniform over the wave
+    const WaveOrder &o = a.ord;
+    int r = w;
+    p = a.parity0;
+    if (o.npar == 2) {
+      p += r & 1;
+      r >>= 1;
+    }
+    const int xc = r % o.nx;
+    r /= o.nx;
+    const int yy = r % o.ty;
+    r /= o.ty;
+    const int zz = r % o.tz;
+    r /= o.tz;
+    const int t = r % g.lt;
+    r /= g.lt;
+    const int y = (r % o.nyt) * o.ty + yy, z = (r / o.nyt) * o.tz + zz;
+    base = ((t * g.lz + z) * g.ly + y) * g.lxh + xc * 16;
+  } else {
+    base = w * 16;
+    p = a.parity0 + blockIdx.y;
+    if (base >= g.vh) return;  // uniform over the wave
+  }
   const int gq = lane >> 4;
   int i = base + (lane & 15);
   const bool ok = i < g.vh;
@@ -316,7 +359,7 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
       dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
       di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
-  const int cin = a.cin;
+  const int cin = CI > 0 ? CI : a.cin;
   // terms that read the zero parity are skipped: hop terms read parity 1 - p,
   // the site term parity p
   int t_lo = 0, t_hi = 9;
@@ -391,23 +434,38 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
     }
     const int qt = t * cin;
     const int qn = qt + cin < nq ? qt + cin : nq - 1;  // first step of the next term
-    int ci = 0;
+    // the scheduling barriers keep each prefetch ahead of the step it overlaps
+    if constexpr (CI > 0 && CI % 2 == 0) {
+      // a compile-time even channel count: the term is straight-line code, so
+      // each step waits for exactly its own operands
+#pragma unroll
+      for (int ci = 0; ci < CI; ci += 2) {
+        load_ops(B, cur, ci + 1, qt + ci + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(A, m);
+        if (ci + 2 < CI) load_ops(A, cur, ci + 2, qt + ci + 2);
+        else load_ops(A, nxt, 0, qn);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(B, m);
+      }
+    } else {
+      int ci = 0;
 #pragma unroll 1
-    for (; ci + 1 < cin; ci += 2) {
-      // the scheduling barriers keep each prefetch ahead of the step it overlaps
-      load_ops(B, cur, ci + 1, qt + ci + 1);
-      __builtin_amdgcn_sched_barrier(0);
-      compute(A, m);
-      const bool here = ci + 2 < cin;
-      load_ops(A, here ? cur : nxt, here ? ci + 2 : 0, here ? qt + ci + 2 : qn);
-      __builtin_amdgcn_sched_barrier(0);
-      compute(B, m);
-    }
-    if (ci < cin) {  // odd channel count: the last channel of the term
-      load_ops(B, nxt, 0, qn);
-      __builtin_amdgcn_sched_barrier(0);
-      compute(A, m);
-      A = B;
+      for (; ci + 1 < cin; ci += 2) {
+        load_ops(B, cur, ci + 1, qt + ci + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(A, m);
+        const bool here = ci + 2 < cin;
+        load_ops(A, here ? cur : nxt, here ? ci + 2 : 0, here ? qt + ci + 2 : qn);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(B, m);
+      }
+      if (ci < cin) {  // odd channel count: the last channel of the term
+        load_ops(B, nxt, 0, qn);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(A, m);
+        A = B;
+      }
     }
     tg = ng;
     cur = nxt;
@@ -426,9 +484,10 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
   }
 }
 
-template <int MT>
+// CI: input channel count known at compile time (0: a.cin at run time)
+template <int MT, int CI>
 __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const float *Apk) {
-  net_layer_mfma_body<MT>(a, Apk);
+  net_layer_mfma_body<MT, CI>(a, Apk);
 }
 
 // weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
@@ -855,6 +914,7 @@ struct Net : Precond {
   bool use_mfma = true;  // matrix-core layer kernel (QCU_NET_KERNEL=valu selects the vector one)
   DevBuf Apk;       // A operands of the matrix-core kernel for one layer
   int block = 128;  // threads per block of the layer kernel
+  int tile = 8;     // rows per tile side of the matrix-core wave order (0: site order)
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
@@ -946,6 +1006,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   adam_t = 0;
   QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 2 * kApkParts * 64 * 4));
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
+  if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
   // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
   if (const char *e = getenv("QCU_NET_SHAPE")) {
     int c = 0, b = 0;
@@ -978,11 +1039,32 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
     const int total = 9 * cin * mtiles * kApkParts * 64;
     hipLaunchKernelGGL(k_net_pack, dim3((unsigned)ndiv(total, 256)), dim3(256), 0, s, Wl, (float *)Apk.p, cin, cout,
                        mtiles);
-    const int wsites = 16 * (block / 64);  // sites per block
-    dim3 grid((unsigned)ndiv(lat->g.vh, wsites), nparity, 1);
+    const int wpb = block / 64;  // waves per block
+    const Geom &g = lat->g;
+    a.ord = WaveOrder{};
+    dim3 grid((unsigned)ndiv(g.vh, 16 * wpb), nparity, 1);
+    if (tile > 0 && g.lxh % 16 == 0) {
+      auto divisor = [&](int n) {  // largest divisor of n not above the tile size
+        int d = tile < n ? tile : n;
+        while (n % d) --d;
+        return d;
+      };
+      a.ord.tiled = 1;
+      a.ord.nx = g.lxh / 16;
+      a.ord.ty = divisor(g.ly);
+      a.ord.tz = divisor(g.lz);
+      a.ord.nyt = g.ly / a.ord.ty;
+      a.ord.npar = nparity;
+      a.ord.nwaves = nparity * (g.vh / 16);
+      grid = dim3((unsigned)ndiv(a.ord.nwaves, wpb), 1, 1);
+    }
     const float *ap = (const float *)Apk.p;
-    if (mtiles == 1) hipLaunchKernelGGL(k_net_layer_mfma<1>, grid, dim3(block), 0, s, a, ap);
-    else hipLaunchKernelGGL(k_net_layer_mfma<2>, grid, dim3(block), 0, s, a, ap);
+    const int ci = cin == 8 ? 8 : cin == 1 ? 1 : 0;
+#define QCU_NET_MFMA_CASE(MT_, CI_) \
+  if (mtiles == MT_ && ci == CI_) hipLaunchKernelGGL((k_net_layer_mfma<MT_, CI_>), grid, dim3(block), 0, s, a, ap);
+    QCU_NET_MFMA_CASE(1, 0) QCU_NET_MFMA_CASE(1, 1) QCU_NET_MFMA_CASE(1, 8)
+    QCU_NET_MFMA_CASE(2, 0) QCU_NET_MFMA_CASE(2, 1) QCU_NET_MFMA_CASE(2, 8)
+#undef QCU_NET_MFMA_CASE
     QCU_HIP(hipGetLastError());
     return 0;
   }
