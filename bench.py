@@ -209,7 +209,7 @@ def main():
         "training": {"seconds": round(train_s, 2), "sources": args.train_sources,
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
-        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_mfma<2>)", "bound": "mfma",
+        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_mfma<2, 8>)", "bound": "mfma",
                      "compute": "FP32 MFMA v_mfma_f32_16x16x4_f32", "achieved": round(tflops, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_TFLOPS, 3),
                      "traffic": net_traffic,
@@ -218,6 +218,10 @@ def main():
                          "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,
                          "hbm_GBps": None if traffic is None else round(traffic / (ms_hop * 1e-3) / 1e9, 1),
+                         "hbm_frac": None if traffic is None else round(traffic / (ms_hop * 1e-3) / 1e9 / PEAK_GBS, 3),
+                         "note": "algorithmic bytes count each of the 8 neighbour spinors per site; neighbours "
+                                 "shared through L2 make the measured HBM traffic smaller, so frac can exceed 1; "
+                                 "hbm_frac is measured traffic / time / peak",
                          "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
         "hop_fp64_wilson": {"ms_per_launch": round(ms64_plain, 5),
                             "achieved_GBps": round(HOP_BYTES[0] * vh / (ms64_plain * 1e-3) / 1e9, 1),
