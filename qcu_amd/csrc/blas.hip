@@ -9,6 +9,10 @@
 // include/lattice_bistabcg.h:149-163).  Vectors may be FP64, FP32 or FP16 with
 // a per-site scale; arithmetic is FP32 for the sloppy precisions and FP64
 // otherwise.  A system whose state says it has stopped is skipped.
+//
+// Every site kernel runs in 256-thread blocks and says so (__launch_bounds__):
+// without it the compiler assumes 1024-thread blocks, caps a thread at 128
+// VGPRs, and the FP64 kernels that hold five 12-component spinors spill.
 #include "qcu_batch.h"
 #include "qcu_internal.h"
 #include "qcu_reduce.h"
@@ -115,7 +119,7 @@ QCU_D void fin_reliable(SysState *st, double rre, double rim, double rr) {
 // ---------------------------------------------------------------- kernels
 // r = b (converted), r~ = r, p = v = 0, x = 0; |r|^2 per system
 template <int Pb, int P, int Px>
-__global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, SpinorPtr b,
+__global__ void __launch_bounds__(256) k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v, SpinorPtr x, SpinorPtr b,
                             const SegDesc *segs, int nseg, double *partials) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.sd) return;
@@ -141,7 +145,7 @@ __global__ void k_bicg_init(SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr v,
 
 // p = r + beta (p - omega v)
 template <int P>
-__global__ void k_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
+__global__ void __launch_bounds__(256) k_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.active) return;
   typedef RT<P> T;
@@ -157,7 +161,7 @@ __global__ void k_p_update(SpinorPtr p, SpinorPtr r, SpinorPtr v, const SegDesc 
 
 // s = r - alpha v
 template <int P>
-__global__ void k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
+__global__ void __launch_bounds__(256) k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, const SegDesc *segs, int nseg, const SysState *st) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.active) return;
   typedef RT<P> T;
@@ -173,7 +177,7 @@ __global__ void k_s_update(SpinorPtr s, SpinorPtr r, SpinorPtr v, const SegDesc 
 // x += alpha P + omega S ; r = s - omega t ; <r~, r>, |r|^2 per system
 // (P, S) = (p, s) for the plain method, (K p, K s) with a right preconditioner K
 template <int Px, int P, int PC>
-__global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s,
+__global__ void __launch_bounds__(256) k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p, SpinorPtr sx, SpinorPtr s,
                             SpinorPtr t, const SegDesc *segs, int nseg, SysState *st, double *partials) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.sd) return;
@@ -222,7 +226,7 @@ __global__ void k_xr_update(SpinorPtr x, SpinorPtr r, SpinorPtr rt, SpinorPtr p,
 // two-vector reductions: <a, b> with |a|^2 (omega of the preconditioned method)
 // or with |b|^2 (rho after a reliable update)
 template <int P, int NORM_OF_A>
-__global__ void k_dot2(SpinorPtr a_, SpinorPtr b_, const SegDesc *segs, int nseg, const SysState *st,
+__global__ void __launch_bounds__(256) k_dot2(SpinorPtr a_, SpinorPtr b_, const SegDesc *segs, int nseg, const SysState *st,
                        double *partials) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.sd) return;
@@ -244,7 +248,7 @@ __global__ void k_dot2(SpinorPtr a_, SpinorPtr b_, const SegDesc *segs, int nseg
 
 // y = a x + b y (all systems)
 template <int Px, int Py>
-__global__ void k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, const SegDesc *segs, int nseg) {
+__global__ void __launch_bounds__(256) k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, const SegDesc *segs, int nseg) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.valid) return;
   typedef RT<Py> T;
@@ -260,7 +264,7 @@ __global__ void k_axpby(double a, SpinorPtr x, double b, SpinorPtr y, const SegD
 }
 
 // r = b - r (FP64), |r|^2 per system
-__global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, const SegDesc *segs, int nseg, double *partials) {
+__global__ void __launch_bounds__(256) k_xmy_norm(SpinorPtr b, SpinorPtr r, const SegDesc *segs, int nseg, double *partials) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.sd) return;
   double acc[1] = {0};
@@ -279,7 +283,7 @@ __global__ void k_xmy_norm(SpinorPtr b, SpinorPtr r, const SegDesc *segs, int ns
 }
 
 template <int P>
-__global__ void k_norm2(SpinorPtr x, const SegDesc *segs, int nseg, double *partials) {
+__global__ void __launch_bounds__(256) k_norm2(SpinorPtr x, const SegDesc *segs, int nseg, double *partials) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.sd) return;
   double acc[1] = {0};
@@ -294,7 +298,7 @@ __global__ void k_norm2(SpinorPtr x, const SegDesc *segs, int nseg, double *part
 
 // out = C (x + a y) in FP64; a = kappa of the segment when a_is_kappa;
 // which: 0 identity, 1 inverse clover of the parity, 2 clover of the parity
-__global__ void k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, int a_is_kappa, SpinorPtr y, int which, int parity,
+__global__ void __launch_bounds__(256) k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, int a_is_kappa, SpinorPtr y, int which, int parity,
                           const SegDesc *segs, int nseg) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.valid) return;
@@ -322,7 +326,7 @@ __global__ void k_cl_axpy(SpinorPtr out, SpinorPtr x, double a, int a_is_kappa, 
 }
 
 // |b - (A x - kappa t)|^2 and |b|^2 per system on one parity
-__global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, const SegDesc *segs, int nseg,
+__global__ void __launch_bounds__(256) k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, const SegDesc *segs, int nseg,
                            double *partials) {
   const TCtx c = tctx(segs, nseg, nullptr);
   if (!c.sd) return;
@@ -354,7 +358,7 @@ __global__ void k_residual(SpinorPtr b, SpinorPtr x, SpinorPtr t, int parity, co
 
 // ---------------------------------------------------------------- CG on the normal equations (FP64)
 // x += alpha p ; res -= alpha q ; |res|^2 per system
-__global__ void k_cg_xr(SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, const SegDesc *segs, int nseg,
+__global__ void __launch_bounds__(256) k_cg_xr(SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, const SegDesc *segs, int nseg,
                         const SysState *st, double *partials) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.sd) return;
@@ -379,7 +383,7 @@ __global__ void k_cg_xr(SpinorPtr x, SpinorPtr r, SpinorPtr p, SpinorPtr q, cons
 }
 
 // p = z + beta p
-__global__ void k_cg_p(SpinorPtr p, SpinorPtr z, const SegDesc *segs, int nseg, const SysState *st) {
+__global__ void __launch_bounds__(256) k_cg_p(SpinorPtr p, SpinorPtr z, const SegDesc *segs, int nseg, const SysState *st) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.active) return;
   const double be = st[c.sys].beta[0];
@@ -394,7 +398,7 @@ __global__ void k_cg_p(SpinorPtr p, SpinorPtr z, const SegDesc *segs, int nseg, 
 }
 
 // out = gamma5 Ao^-1 in on the odd sites (gamma5 = diag(1, 1, -1, -1) in this basis)
-__global__ void k_g5_clinv(SpinorPtr out, SpinorPtr in, const SegDesc *segs, int nseg, const SysState *st) {
+__global__ void __launch_bounds__(256) k_g5_clinv(SpinorPtr out, SpinorPtr in, const SegDesc *segs, int nseg, const SysState *st) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.active) return;
   cplx<double> v[12];
@@ -414,7 +418,7 @@ __global__ void k_g5_clinv(SpinorPtr out, SpinorPtr in, const SegDesc *segs, int
 }
 
 // z = r - kappa^2 gamma5 y ; |z|^2 per system
-__global__ void k_cg_z(SpinorPtr z, SpinorPtr r, SpinorPtr y, const SegDesc *segs, int nseg, const SysState *st,
+__global__ void __launch_bounds__(256) k_cg_z(SpinorPtr z, SpinorPtr r, SpinorPtr y, const SegDesc *segs, int nseg, const SysState *st,
                        double *partials) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.sd) return;
@@ -436,7 +440,7 @@ __global__ void k_cg_z(SpinorPtr z, SpinorPtr r, SpinorPtr y, const SegDesc *seg
 }
 
 // |q|^2 per system of the running systems
-__global__ void k_cg_norm(SpinorPtr q, const SegDesc *segs, int nseg, const SysState *st, double *partials) {
+__global__ void __launch_bounds__(256) k_cg_norm(SpinorPtr q, const SegDesc *segs, int nseg, const SysState *st, double *partials) {
   const TCtx c = tctx(segs, nseg, st);
   if (!c.sd) return;
   double acc[1] = {0};

@@ -203,7 +203,7 @@ QCU_D void pack_store(double2 *dst, int vh, int i, const cd (&m)[36], int b) {
 }
 
 // clov: packed A (may be null), cinv: packed A^-1, both double2 [36][Vh]
-__global__ void k_clover(LinkPtr U, Geom g, int parity, double coeff, double2 *clov, double2 *cinv,
+__global__ void __launch_bounds__(128) k_clover(LinkPtr U, Geom g, int parity, double coeff, double2 *clov, double2 *cinv,
                          int *bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.vh) return;
@@ -238,7 +238,7 @@ __global__ void k_clover(LinkPtr U, Geom g, int parity, double coeff, double2 *c
 }
 
 // FP64 packed clover -> FP32 / FP16 (+ per-site scale)
-__global__ void k_clover_cvt(const double2 *src, int vh, int prec, void *dst, float *norm) {
+__global__ void __launch_bounds__(128) k_clover_cvt(const double2 *src, int vh, int prec, void *dst, float *norm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   double v[72];

@@ -12,7 +12,7 @@ namespace qcu {
 
 // one thread per (p, mu, site): rows 0 and 1 of the 3x3 link
 template <int P>
-__global__ void k_gauge_in(const double2 *ext, Geom g, void *a, void *b) {
+__global__ void __launch_bounds__(256) k_gauge_in(const double2 *ext, Geom g, void *a, void *b) {
   const int vh = g.vh;
   const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= 8L * vh) return;
