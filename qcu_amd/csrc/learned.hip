@@ -126,113 +126,6 @@ QCU_D TermGeo term_geo(const Geom &g, const Site &s, int i, int p, int t) {
   return r;
 }
 
-// v <- ph * U v (fwd) or ph * U^dag v, on each of the four spins
-QCU_D void transport4(const cf (&u)[9], cf (&v)[12], int fwd, float ph) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    cf a[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      cf acc = mk(0.f, 0.f);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (fwd) cmad(acc, u[r * 3 + c], v[s * 3 + c]);
-        else cmadj(acc, u[c * 3 + r], v[s * 3 + c]);
-      }
-      a[r] = mk(ph * acc.re, ph * acc.im);
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) v[s * 3 + r] = a[r];
-  }
-}
-
-// grid: x = site blocks x output-channel groups (group fastest, so that the
-// groups of one site block run together and share its loads in L2; the
-// XCD-aware remap keeps neighbouring site blocks on one L2), y = output parity.
-// CO output channels per thread; CI: input channel count known at compile time
-// (0: a.cin at run time).  The input channel is the outer loop so that the
-// nine neighbour reads of one channel are close in time and mostly hit in L2.
-template <int CO, int CI>
-__global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
-  if (a.st && a.st->done) return;
-  const Geom g = a.g;
-  const int blk = remap_block(blockIdx.x, gridDim.x, 1);
-  const int sb = blk / a.ngroups;
-  const int og = (blk - sb * a.ngroups) * CO;
-  const int i = sb * blockDim.x + threadIdx.x;
-  const int p = a.parity0 + blockIdx.y;
-  if (i >= g.vh) return;
-  const Site site = decode(g, i, p);
-  cf acc[CO][12];
-#pragma unroll
-  for (int o = 0; o < CO; ++o)
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
-  const int cin = CI > 0 ? CI : a.cin, cout = a.cout;
-  const int t_lo = (a.zero_par == 1 - p) ? 8 : 0, t_hi = (a.zero_par == p) ? 8 : 9;
-#pragma unroll 1
-  for (int ci = 0; ci < cin; ++ci) {
-#pragma unroll 1
-    for (int t = t_lo; t < t_hi; ++t) {
-      const TermGeo tg = term_geo(g, site, i, p, t);
-      cf v[12];
-      SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
-      if (tg.hop) {
-        cf u[9];
-        LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
-        transport4(u, v, tg.fwd, tg.ph);
-      }
-#pragma unroll
-      for (int o = 0; o < CO; ++o) {
-        if (og + o >= cout) break;
-        const float2 *w = a.W + (((size_t)t * cout + og + o) * cin + ci) * 16;
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-          for (int s2 = 0; s2 < 4; ++s2) {
-            const float2 ww = w[s * 4 + s2];
-            const cf wc = mk(ww.x, ww.y);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) cmad(acc[o][s * 3 + c], wc, v[s2 * 3 + c]);
-          }
-      }
-    }
-  }
-#pragma unroll
-  for (int o = 0; o < CO; ++o) {
-    if (og + o >= cout) break;
-    SpinorIO<P32>::store(chan_field(a.out, og + o, p, g.vh), i, acc[o]);
-  }
-}
-
-// ---- matrix-core layer -----------------------------------------------------
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-// A operands of one layer: Apk[((t * cin + i) * MT + mt) * 3 + part][lane],
-// part 0 = Re W, 1 = Im W, 2 = -Im W, row m = lane % 16 -> (o = 4 mt + m / 4,
-// s = m % 4), column k = lane / 16 -> s2
-constexpr int kApkParts = 3;
-__global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int mtiles) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int total = 9 * cin * mtiles * kApkParts * 64;
-  if (idx >= total) return;
-  const int lane = idx & 63;
-  int r = idx >> 6;
-  const int part = r % kApkParts;
-  r /= kApkParts;
-  const int mt = r % mtiles;
-  r /= mtiles;
-  const int i = r % cin, t = r / cin;
-  const int ml = lane & 15, s2 = lane >> 4;
-  const int o = 4 * mt + (ml >> 2), sp = ml & 3;
-  float v = 0.f;
-  if (o < cout) {
-    const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + sp * 4 + s2];
-    v = part == 0 ? w.x : part == 1 ? w.y : -w.y;
-  }
-  Apk[idx] = v;
-}
-
 // the colour matrix of a term, applied as v <- M v: M = ph U for a forward
 // hop and ph U^dag for a backward one, from the two stored rows of U; the
 // identity for the site term
@@ -279,6 +172,139 @@ QCU_D void transport1(const cf (&m)[9], cf (&v)[3]) {
   }
 #pragma unroll
   for (int r = 0; r < 3; ++r) v[r] = a[r];
+}
+
+// the colour matrix of term tg at this site (FP32 links)
+QCU_D void term_link_matrix(const LinkPtr &U, const TermGeo &tg, cf (&m)[9]) {
+  float4 raw[3];
+  if (tg.hop) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) raw[k] = ld_at<float4>(U.a, ((size_t)tg.slab * 3 + k) * U.vh * 16, (unsigned)tg.li * 16u);
+  }
+  term_matrix(raw, tg.hop, tg.fwd, tg.ph, m);
+}
+
+// v <- M v on each of the four spins of a spinor
+QCU_D void transport_spins(const cf (&m)[9], cf (&v)[12]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    cf w[3] = {v[s * 3], v[s * 3 + 1], v[s * 3 + 2]};
+    transport1(m, w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[s * 3 + c] = w[c];
+  }
+}
+
+// grid: x = site blocks x output-channel groups (group fastest, so that the
+// groups of one site block run together and share its loads in L2; the
+// XCD-aware remap keeps neighbouring site blocks on one L2), y = output parity.
+// CO output channels per thread; CI: input channel count known at compile time
+// (0: a.cin at run time).  The term is the outer loop.  The colour transport and
+// the spin matrices act on different indices, so they commute: with fewer
+// output than input channels the input channels are first mixed at the
+// neighbour and each output channel is transported once per term
+// (sum_i W_i (M v_i) = M sum_i W_i v_i), which saves (cin - CO) / cin of the
+// transports of the 8 -> 1 layer.
+template <int CO, int CI>
+__global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
+  if (a.st && a.st->done) return;
+  const Geom g = a.g;
+  const int blk = remap_block(blockIdx.x, gridDim.x, 1);
+  const int sb = blk / a.ngroups;
+  const int og = (blk - sb * a.ngroups) * CO;
+  const int i = sb * blockDim.x + threadIdx.x;
+  const int p = a.parity0 + blockIdx.y;
+  if (i >= g.vh) return;
+  const Site site = decode(g, i, p);
+  cf acc[CO][12];
+#pragma unroll
+  for (int o = 0; o < CO; ++o)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[o][k] = mk(0.f, 0.f);
+  const int cin = CI > 0 ? CI : a.cin, cout = a.cout;
+  const int t_lo = (a.zero_par == 1 - p) ? 8 : 0, t_hi = (a.zero_par == p) ? 8 : 9;
+  // one output channel per thread: mixing first costs no extra live accumulators
+  const bool mix_first = CO == 1 && cin > 1;
+  // acc_o[s][c] += sum_s2 W[t][o][ci][s][s2] v[s2][c]
+  auto mix = [&](cf (&dst)[CO][12], const cf (&v)[12], int t, int ci) {
+#pragma unroll
+    for (int o = 0; o < CO; ++o) {
+      if (og + o >= cout) break;
+      const float2 *w = a.W + (((size_t)t * cout + og + o) * cin + ci) * 16;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) {
+          const float2 ww = w[s * 4 + s2];
+          const cf wc = mk(ww.x, ww.y);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) cmad(dst[o][s * 3 + c], wc, v[s2 * 3 + c]);
+        }
+    }
+  };
+#pragma unroll 1
+  for (int t = t_lo; t < t_hi; ++t) {
+    const TermGeo tg = term_geo(g, site, i, p, t);
+    cf m[9];
+    term_link_matrix(a.U, tg, m);
+    if constexpr (CO == 1) {
+      if (mix_first) {
+        cf tmp[1][12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) tmp[0][k] = mk(0.f, 0.f);
+#pragma unroll 1
+        for (int ci = 0; ci < cin; ++ci) {
+          cf v[12];
+          SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
+          mix(tmp, v, t, ci);
+        }
+        if (tg.hop) transport_spins(m, tmp[0]);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[0][k] = acc[0][k] + tmp[0][k];
+        continue;
+      }
+    }
+#pragma unroll 1
+    for (int ci = 0; ci < cin; ++ci) {
+      cf v[12];
+      SpinorIO<P32>::load(chan_field(a.in, ci, tg.jp, g.vh), tg.j, v);
+      if (tg.hop) transport_spins(m, v);
+      mix(acc, v, t, ci);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < CO; ++o) {
+    if (og + o >= cout) break;
+    SpinorIO<P32>::store(chan_field(a.out, og + o, p, g.vh), i, acc[o]);
+  }
+}
+
+// ---- matrix-core layer -----------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// A operands of one layer: Apk[((t * cin + i) * MT + mt) * 3 + part][lane],
+// part 0 = Re W, 1 = Im W, 2 = -Im W, row m = lane % 16 -> (o = 4 mt + m / 4,
+// s = m % 4), column k = lane / 16 -> s2
+constexpr int kApkParts = 3;
+__global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int mtiles) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = 9 * cin * mtiles * kApkParts * 64;
+  if (idx >= total) return;
+  const int lane = idx & 63;
+  int r = idx >> 6;
+  const int part = r % kApkParts;
+  r /= kApkParts;
+  const int mt = r % mtiles;
+  r /= mtiles;
+  const int i = r % cin, t = r / cin;
+  const int ml = lane & 15, s2 = lane >> 4;
+  const int o = 4 * mt + (ml >> 2), sp = ml & 3;
+  float v = 0.f;
+  if (o < cout) {
+    const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + sp * 4 + s2];
+    v = part == 0 ? w.x : part == 1 ? w.y : -w.y;
+  }
+  Apk[idx] = v;
 }
 
 // The channel and spin mixing of one layer is a complex GEMM per colour,
@@ -519,9 +545,9 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
     cf v[12];
     SpinorIO<P32>::load(chan_field(a.act, ci, tg.jp, g.vh), tg.j, v);
     if (tg.hop) {
-      cf u[9];
-      LinkIO<P32>::load(a.U, tg.slab, tg.li, u);
-      transport4(u, v, tg.fwd, tg.ph);
+      cf m[9];
+      term_link_matrix(a.U, tg, m);
+      transport_spins(m, v);
     }
 #pragma unroll
     for (int o = 0; o < CO; ++o) {
@@ -634,13 +660,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     // columns: transported inputs of channels 4 nt + n / 4, spin n % 4
     const TermGeo tg = term_geo(g, site, i, p, t);
     cf u[9];
-    if (tg.hop) {
-      float4 raw[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        raw[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * a.U.vh * 16, (unsigned)tg.li * 16u);
-      term_matrix(raw, 1, tg.fwd, tg.ph, u);
-    }
+    if (tg.hop) term_link_matrix(a.U, tg, u);
     cf vv[NT][3];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
