@@ -1,6 +1,7 @@
 """Propagators from one 12-right-hand-side batch: every column is the solution
 of its spin-colour source (oracle FP64 true residual), equal to the single
-solve, and carries PyQuda's 2 kappa mass normalisation."""
+solve, and carries PyQuda's 2 kappa mass normalisation; the single-fermion
+invert / invertPC of qcu_amd.general carry it too."""
 import numpy as np
 import pytest
 import torch
@@ -48,3 +49,24 @@ def test_invert12_matches_invert(orc):
             b12[..., :, s, :, c] = prop.wall(L, 2, s, c)
     P12, _ = prop.invert12(b12, lat, prec=FP64, normalization="kappa")
     assert torch.equal(P, P12)
+
+
+def test_general_invert_and_invert_pc(orc):
+    """general.invert / invertPC: 2 kappa x with x the oracle-checked solution;
+    the explicit even-odd form gives the same field"""
+    from qcu_amd import general
+    from qcu_amd.lattice import Lattice
+    L, kappa, csw = (4, 4, 4, 8), 0.124, 1.0
+    U = orc.gauge_random(L, 0.3, 172)
+    lat = Lattice(L, kappa, csw, -1)
+    lat.load_gauge(dev(U))
+    b = orc.gauss((2, orc.vol(L) // 2, 4, 3), 173)
+    y, info = general.invert(dev(b), lat)
+    y_pc, _ = general.invertPC(dev(b), lat)
+    y_cg, _ = general.invert(dev(b), lat, method="cgnr")
+    tr = orc.true_residual(host(y) / (2 * kappa), b, U, L, kappa, csw, -1)
+    tr_cg = orc.true_residual(host(y_cg) / (2 * kappa), b, U, L, kappa, csw, -1)
+    log(f"invert: {info.iters} iters, oracle true residual of y / 2 kappa {tr:.2e} (cgnr {tr_cg:.2e})")
+    assert tr <= 1e-10 and tr_cg <= 1e-10
+    assert torch.equal(y, y_pc)
+    assert rel(host(y_cg), host(y)) < 1e-8

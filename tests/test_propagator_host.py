@@ -35,3 +35,33 @@ def test_propagator_shape_and_errors():
         assert "not supported" in str(e)
     else:
         raise AssertionError("unsupported source type accepted")
+
+
+class _FakeLattice:
+    """stands in for a Lattice: returns b as the solution with a chosen convergence flag"""
+    kappa = 0.125
+
+    def __init__(self, converged):
+        self.converged = converged
+        self.calls = []
+
+    def solve(self, b, **kw):
+        from qcu_amd.lattice import SolveResult
+        self.calls.append(kw)
+        return b.clone(), SolveResult(3, 0, self.converged, False, 1e-11, 0.0, 0)
+
+
+def test_general_invert_normalisation_and_failure():
+    from qcu_amd import general
+    b = torch.ones((2, 2, 2, 2, 1, 4, 3), dtype=torch.complex128)
+    lat = _FakeLattice(True)
+    y, info = general.invert(b, lat, method="cgnr")
+    assert torch.allclose(y, b * 0.25) and info.iters == 3 and lat.calls[0]["method"] == "cgnr"
+    y_pc, _ = general.invertPC(b, lat)
+    assert torch.equal(y_pc, y)
+    try:
+        general.invert(b, _FakeLattice(False))
+    except RuntimeError as e:
+        assert "did not reach" in str(e)
+    else:
+        raise AssertionError("a solve that did not converge was accepted")
