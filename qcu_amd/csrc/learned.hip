@@ -14,12 +14,19 @@
 // vector) is embedded in the full lattice with zero even sites, and the odd
 // part of the single output channel is kept.
 //
-// Kernels: layer forward (one thread per site and group of output channels,
-// weights wavefront-uniform so they come through scalar loads), the data
-// backward is the forward kernel with conjugate-transposed weights and
-// swapped hop directions, the weight gradient is a two-stage deterministic
-// reduction of g_o(x) (T_t a_i)(x)^dag over sites and colours, and Adam runs on
-// the device.  Training minimises |Mhat K b - b|^2 / |b|^2.
+// Kernels:
+// * layer forward on the matrix cores (k_net_layer_mfma: the channel and spin
+//   mixing as real FP32 MFMA products, one colour transport per lane and step)
+//   for layers with several output channels, and on the vector units
+//   (k_net_layer, one thread per site and group of output channels) for the
+//   single-output layer and as a cross-check;
+// * the data backward is the forward kernel with conjugate-transposed weights
+//   and swapped hop directions;
+// * the weight gradient is a deterministic two-stage reduction of
+//   g_o(x) (T_t a_i)(x)^dag over sites and colours (matrix cores, with a
+//   vector cross-check);
+// * Adam runs on the device.
+// Training minimises |Mhat K b - b|^2 / |b|^2.
 #include <atomic>
 #include <chrono>
 #include <cmath>
