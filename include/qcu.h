@@ -200,7 +200,9 @@ int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss);
 /* mean loss over the sources, no update */
 int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss);
 /* ms per launch of one layer's forward kernel (both parities) and its
- * algorithmic flops per launch, timed with HIP events on the lattice stream */
+ * algorithmic flops per launch, timed with HIP events on the lattice stream;
+ * the flops count 8 per complex multiply-add, 6 in the matrix-core channel
+ * mixing (three real products per complex product) */
 int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *flops);
 /* out = K in on one-parity (odd) external fields, as applied inside the solver */
 int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);

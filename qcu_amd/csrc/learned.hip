@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) k_net_layer(NetLayerArgs a) {
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // A operands of one layer: Apk[((t * cin + i) * MT + mt) * 3 + part][lane],
-// part 0 = Re W, 1 = Im W, 2 = -Im W, row m = lane % 16 -> (o = 4 mt + m / 4,
+// part 0 = Re W, 1 = Im W, 2 = Re W + Im W, row m = lane % 16 -> (o = 4 mt + m / 4,
 // s = m % 4), column k = lane / 16 -> s2
 constexpr int kApkParts = 3;
 __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int mtiles) {
@@ -309,7 +309,7 @@ __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int m
   float v = 0.f;
   if (o < cout) {
     const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + sp * 4 + s2];
-    v = part == 0 ? w.x : part == 1 ? w.y : -w.y;
+    v = part == 0 ? w.x : part == 1 ? w.y : w.x + w.y;
   }
   Apk[idx] = v;
 }
@@ -317,8 +317,12 @@ __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int m
 // The channel and spin mixing of one layer is a complex GEMM per colour,
 //   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
 //   k = (term t, input channel i, spin s2),  n = site,
-// done as four real products on v_mfma_f32_16x16x4_f32 in FP32:
-//   Re D += Wr Vr + (-Wi) Vi,  Im D += Wi Vr + Wr Vi.
+// done as three real products on v_mfma_f32_16x16x4_f32 in FP32 (the
+// three-multiplication complex product):
+//   P1 += Wr Vr,  P2 += Wi Vi,  P3 += (Wr + Wi)(Vr + Vi),
+//   Re D = P1 - P2,  Im D = P3 - P1 - P2,
+// with Wr + Wi packed beside the weights and Vr + Vi formed per step; three
+// products instead of four cut the matrix-core work by a quarter.
 // Lane l of an MFMA step holds A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
 // and returns D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.  A K step is the four
 // spins s2 of one (t, i): lane group g = l / 16 transports spin g of its site,
@@ -334,7 +338,7 @@ __global__ void k_net_pack(const float2 *W, float *Apk, int cin, int cout, int m
 // so each step waits only for its own operands.
 struct MfmaOps {
   float2 s[3];        // the lane's spin: colours 0..2
-  float w[2][3];      // [mt][Re W, Im W, -Im W]
+  float w[2][3];      // [mt][Re W, Im W, Re W + Im W]
 };
 
 template <int MT, int CI>
@@ -384,13 +388,14 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
     const int f = 6 * gq + 2 * k;
     poff[k] = (unsigned)(f >> 2) * vh16 + (unsigned)(f & 3) * 4u;
   }
-  f4v dr[3][MT], di[3][MT];
+  f4v p1[3][MT], p2[3][MT], p3[3][MT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      dr[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
-      di[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      p1[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      p2[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      p3[c][mt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
   const int cin = CI > 0 ? CI : a.cin;
   // terms that read the zero parity are skipped: hop terms read parity 1 - p,
@@ -436,13 +441,12 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
     // are computed and never stored
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float br = v[c].re, bi = v[c].im;
+      const float br = v[c].re, bi = v[c].im, bs = br + bi;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][0], br, dr[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][1], br, di[c][mt], 0, 0, 0);
-        dr[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][2], bi, dr[c][mt], 0, 0, 0);
-        di[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][0], bi, di[c][mt], 0, 0, 0);
+        p1[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][0], br, p1[c][mt], 0, 0, 0);
+        p2[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][1], bi, p2[c][mt], 0, 0, 0);
+        p3[c][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w[mt][2], bs, p3[c][mt], 0, 0, 0);
       }
     }
   };
@@ -512,7 +516,10 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[j * 3 + c] = mk(dr[c][mt][j], di[c][mt][j]);
+      for (int c = 0; c < 3; ++c) {
+        const float a = p1[c][mt][j], b = p2[c][mt][j];
+        out[j * 3 + c] = mk(a - b, p3[c][mt][j] - a - b);
+      }
     SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
   }
 }
@@ -1535,8 +1542,12 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   if (flops) {
     // per output site: 9 terms x cout x cin spin-matrix products on 3 colours
     // (16 complex multiply-adds each) + 8 hops x cin transports of 4 spins
-    // (9 complex multiply-adds each); 8 real flops per complex multiply-add
-    const double per_site = 8.0 * (9.0 * co * ci * 16 * 3 + 8.0 * ci * 4 * 9);
+    // (9 complex multiply-adds each).  A complex multiply-add is 8 real flops,
+    // except in the matrix-core mixing, which uses three real products per
+    // complex product (6 flops)
+    const bool mfma = net.use_mfma && co > 1;
+    const double mix = (mfma ? 6.0 : 8.0) * 9.0 * co * ci * 16 * 3;
+    const double per_site = mix + 8.0 * (8.0 * ci * 4 * 9);
     *flops = per_site * 2.0 * net.lat->g.vh;
   }
   return 0;
