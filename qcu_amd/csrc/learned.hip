@@ -977,6 +977,9 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_CHECK(l && l->gauge_loaded, QCU_ERR_STATE, "net: lattice has no gauge field");
   QCU_CHECK(nlayers >= 1 && nlayers <= kNetMaxLayers, QCU_ERR_ARG, "net: 1 to 8 layers");
   QCU_CHECK((l->prec_mask >> P32) & 1, QCU_ERR_STATE, "net: FP32 links not resident");
+  // the layer kernels address one channel field of one parity (96 bytes per
+  // site) with 32-bit per-lane byte offsets
+  QCU_CHECK((size_t)l->g.vh * 96 < ((size_t)1 << 32), QCU_ERR_DIMS, "net: lattice too large (Vh * 96 bytes >= 4 GiB)");
   lat = l;
   nl = nlayers;
   id = ++g_net_ids;
