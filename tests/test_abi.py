@@ -2,6 +2,8 @@
 
 No compute call is made (there is no GPU in the build container); the
 reference-named entry points must match include/qcu.h:21-30 of the reference.
+Argument errors are checked before any device call, so their codes and
+messages are tested here too.
 """
 import ctypes as C
 import os
@@ -81,3 +83,26 @@ def test_pointer_shim():
     assert address(Pointer("void", 1234)) == 1234
     with pytest.raises(ValueError):
         ndarrayDataPointer([1, 2, 3])
+
+
+def test_argument_errors_are_reported_without_a_device(qlib):
+    """entry points validate their arguments before any device call: codes of
+    include/qcu.h, a message, and no exception or exit across the ABI"""
+    from qcu_amd.lattice import Lattice
+    ERR_ARG, ERR_DIMS = 1, 2
+    for L, tb, code, words in (((3, 4, 4, 4), -1, ERR_DIMS, "Lx must be even"),
+                               ((4, 0, 4, 4), -1, ERR_DIMS, "positive"),
+                               ((4, 4, 4, 4), 0, ERR_ARG, "t_boundary"),
+                               ((2048, 2048, 1024, 1024), -1, ERR_DIMS, "too large")):
+        with pytest.raises(qlib.QcuError) as e:
+            Lattice(L, 0.12, 0.0, tb)
+        assert e.value.code == code and words in str(e.value)
+    lib = qlib.load()
+    null = C.c_void_p()
+    assert lib.qcuSolve(null, null, null, None, None) == ERR_ARG
+    assert lib.qcuLatticeLoadGauge(null, null) == ERR_ARG
+    assert lib.qcuNetForward(null, null, null) == ERR_ARG
+    assert lib.qcuBatchSolve(null, None, None, None, None) == ERR_ARG
+    assert lib.qcuGetLastError() == ERR_ARG and lib.qcuGetLastErrorString()
+    chan = (C.c_int * 3)(1, 8, 1)
+    assert lib.qcuNetCreate(C.byref(C.c_void_p()), null, 2, chan, 1, 0.05) == ERR_ARG
