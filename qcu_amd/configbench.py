@@ -1,6 +1,6 @@
 """Benchmarks of the smaller workloads (time-to-solution at 1e-10 true residual).
 
-    python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b] [c4]
+    python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b] [c4] [legacy]
 
 c1  plain clover BiCGSTAB FP64, 16^3x32, eps 0.3, csw 1.0, kappa 0.124, point
     source (origin, spin 0, colour 0): iterations, time, FP64 hop GB/s.
@@ -14,6 +14,12 @@ c3b 16 Wilson lattices (4 each of 8^4, 12^4, 16^3x32, 20^3x40; eps 0.3,
 c4  32^3x64 clover, eps 0.2, csw 1.0, kappa 0.1248, Gaussian source: plain
     FP64 and mixed FP16 (reliable-update drop 0.01) solves, no network (the
     solver part of the headline benchmark, e.g. for profiling).
+legacy  the reference's entry points through qcu_amd.pyqcu on the c1 lattice
+    (16^3x32, eps 0.3): dslashQcu, dslashCloverQcu and ncclDslashQcu per call
+    (first call and steady state: the first call of a gauge pointer converts
+    the links and, for clover, builds the clover term), against the resident
+    FP64 hop of the handle API; ncclBistabCgQcu (kappa 0.125, periodic, stop
+    on Re<r,r> < 1e-9) with its |b - M x|^2.
 
 Every solve must reach |b - M x| / |b| <= 1e-10 (FP64); one JSON line per
 workload.  Times are medians of 3 (2 for batches) after a warm-up solve.
@@ -146,9 +152,50 @@ def c4():
     return d
 
 
+def legacy():
+    from . import pyqcu
+    from .lattice import fermion_shape
+    L = (16, 16, 16, 32)
+    U = random_gauge(L, 0.3, 12)
+    param = pyqcu.QcuParam()
+    param.lattice_size = list(L)
+    grid = pyqcu.QcuParam()
+    grid.lattice_size = [1, 1, 1, 1]
+    psi = gaussian(fermion_shape(L, full=False), 13)
+    out = torch.empty_like(psi)
+
+    def per_call(fn, n=7):
+        ts = []
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return ts[0] * 1e3, sorted(ts[1:])[(n - 1) // 2] * 1e3
+
+    d = {}
+    for name, fn in (("dslashQcu", lambda: pyqcu.dslashQcu(out, psi, U, param, 0)),
+                     ("dslashCloverQcu", lambda: pyqcu.dslashCloverQcu(out, psi, U, param, 0)),
+                     ("ncclDslashQcu", lambda: pyqcu.ncclDslashQcu(out, psi, U, param, 1, grid))):
+        first, steady = per_call(fn)
+        d[name] = {"first_ms": first, "steady_ms": steady}
+    lat = Lattice(L, 0.125, 0.0, 1)
+    lat.load_gauge(U)
+    d["resident_fp64_hop_ms"] = lat.bench_hop(prec=FP64, clover=False, xpay=False, iters=50)
+    b = gaussian(fermion_shape(L), 14)
+    x = torch.zeros_like(b)
+    t, _ = timed(lambda: pyqcu.ncclBistabCgQcu(x, b, U, param, grid))
+    # the handle API's operator at the legacy kappa and boundary gives |b - M x|^2
+    r = b - lat.mat(x)
+    d["ncclBistabCgQcu"] = {"tts_ms": t * 1e3, "r2": float(torch.vdot(r.flatten(), r.flatten()).real)}
+    lat.close()
+    return d
+
+
 def main():
     which = sys.argv[1:] or ["c1", "c2", "c3a", "c3b"]
-    fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b, "c4": c4}
+    fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b, "c4": c4, "legacy": legacy}
     for w in which:
         print(json.dumps({"workload": w, **fns[w]()}), flush=True)
 

@@ -57,6 +57,67 @@ int gauge_ext_to_internal(const void *ext, const Geom &g, int prec, void *a, voi
   return 0;
 }
 
+// ---- content hash of an external gauge field -------------------------------
+// A 64-bit summary of every bit of the field, position-dependent, used by the
+// reference-named entry points to keep converted links and the clover term
+// while the caller's gauge is unchanged.  Each complex entry k with bit
+// patterns (a, b) is mixed with k and the mixed words are added modulo 2^64;
+// addition commutes, so the result does not depend on the block order.
+QCU_D unsigned long long hash_mix(unsigned long long a, unsigned long long b, unsigned long long k) {
+  unsigned long long x = a * 0x9E3779B97F4A7C15ull + b * 0xC2B2AE3D27D4EB4Full + k * 0x165667B19E3779F9ull;
+  x ^= x >> 29;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 32;
+  x *= 0x94D049BB133111EBull;
+  return x ^ (x >> 29);
+}
+
+constexpr int kHashBlocks = 1024;
+
+__global__ void __launch_bounds__(256) k_gauge_hash(const double2 *ext, long n, unsigned long long *part) {
+  unsigned long long h = 0;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const double2 v = ext[k];
+    h += hash_mix((unsigned long long)__double_as_longlong(v.x), (unsigned long long)__double_as_longlong(v.y),
+                  (unsigned long long)k);
+  }
+  __shared__ unsigned long long sh[256];
+  sh[threadIdx.x] = h;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_hash_sum(unsigned long long *part, int n) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long h = 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) h += part[k];
+  sh[threadIdx.x] = h;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[n] = sh[0];
+}
+
+// scratch: (kHashBlocks + 1) 64-bit words of device memory
+int gauge_hash(const void *ext, const Geom &g, void *scratch, unsigned long long *out, hipStream_t s) {
+  const long n = 36L * 2 * g.vh;  // complex entries: [4][2][Vh][3][3]
+  unsigned long long *part = (unsigned long long *)scratch;
+  hipLaunchKernelGGL(k_gauge_hash, dim3(kHashBlocks), dim3(256), 0, s, (const double2 *)ext, n, part);
+  hipLaunchKernelGGL(k_hash_sum, dim3(1), dim3(256), 0, s, part, kHashBlocks);
+  QCU_HIP(hipGetLastError());
+  QCU_HIP(hipMemcpyAsync(out, part + kHashBlocks, sizeof(*out), hipMemcpyDeviceToHost, s));
+  QCU_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+size_t gauge_hash_scratch_bytes() { return (kHashBlocks + 1) * sizeof(unsigned long long); }
+
 // external one-parity field [Vh][4][3] <-> system (segment seg, rhs r) of a batch vector
 template <int P>
 __global__ void k_spinor_in(const double2 *ext, int vh, int nrhs, int rhs, SpinorPtr dst) {

@@ -120,6 +120,9 @@ struct BatchOp {
 // --------------------------------------------------------------------------
 // layout.hip
 int gauge_ext_to_internal(const void *ext, const Geom &g, int prec, void *a, void *b, hipStream_t s);
+// 64-bit content hash of an external gauge field (scratch: gauge_hash_scratch_bytes() of device memory)
+int gauge_hash(const void *ext, const Geom &g, void *scratch, unsigned long long *out, hipStream_t s);
+size_t gauge_hash_scratch_bytes();
 // external one-parity field <-> system (seg, rhs) of a batch vector
 int spinor_ext_to_batch(const void *ext, const BatchOp &op, int seg, int rhs, SpinorPtr dst, int prec,
                         hipStream_t s);

@@ -133,6 +133,39 @@ def test_legacy_clover_dslash(orc, cfg0):
         assert err < 1e-15 * cond * 100
 
 
+def test_legacy_gauge_reuse_follows_contents(orc):
+    """the reference-named entry points keep converted links and the clover term
+    while the caller's gauge is unchanged; new contents at the same pointer,
+    or another pointer, are picked up on the next call"""
+    from qcu_amd import pyqcu
+    L = (4, 4, 4, 8)
+    U1, U2 = orc.gauge_random(L, 0.2, 60), orc.gauge_random(L, 0.2, 61)
+    psi = orc.gauss((orc.vol(L) // 2, 4, 3), 62)
+    param = pyqcu.QcuParam()
+    param.lattice_size = list(L)
+    g, inp = dev(U1), dev(psi)
+
+    def run(fn, gauge):
+        out = torch.zeros_like(inp)
+        fn(out, inp, gauge, param, 0)
+        return host(out)
+
+    # kappa*csw = 1: the clover bound scales with the worst block condition number
+    cond = max(np.linalg.cond(orc.clover_build(U, L, 0, 1.0)).max() for U in (U1, U2))
+    for fn, ref_of, tol in ((pyqcu.dslashCloverQcu, lambda U: orc.clover_dslash(psi, U, L, 0, 1.0, 1),
+                             max(1e-12, 1e-15 * cond * 100)),
+                            (pyqcu.dslashQcu, lambda U: orc.hop(psi, U, L, 0, 1), 1e-12)):
+        g.copy_(dev(U1))
+        a, b = run(fn, g), run(fn, g)
+        assert np.array_equal(a, b) and rel(a, ref_of(U1)) < tol
+        g.copy_(dev(U2))  # same pointer, new contents
+        c = run(fn, g)
+        assert rel(c, ref_of(U2)) < tol
+        d = run(fn, dev(U1))  # another pointer
+        assert rel(d, ref_of(U1)) < tol
+        log(f"{fn.__name__}: reuse {rel(a, ref_of(U1)):.1e}, new contents {rel(c, ref_of(U2)):.1e}")
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6), (2, 5e-3)])
 def test_clover_hop(orc, prec, tol):
     from qcu_amd.lattice import Lattice
