@@ -606,8 +606,11 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 //   gW[t][o][i][s][s2] = sum over sites x of both parities and colours c of
 //                        g_o(x)[s][c] conj(v_i(x)[s2][c]),  v_i = T_t a_i,
 // i.e. D[m = (o, s)][n = (i, s2)] = sum_k G[m][k] conj(V)[k][n] over k = (x, c),
-// as four real FP32 products per colour on v_mfma_f32_16x16x4_f32:
-//   Dr += Gr Vr + Gi Vi,  Di += Gi Vr,  Dn += Gr Vi,  Im D = Di - Dn.
+// as three real FP32 products per colour on v_mfma_f32_16x16x4_f32 (the
+// three-multiplication complex product, with G conj(V)):
+//   P1 += Gr Vr,  P2 += Gi Vi,  P3 += (Gr + Gi)(Vr - Vi),
+//   Re D = P1 + P2,  Im D = P3 - P1 + P2
+// (accumulators dr = P1, dn = P2, di = P3).
 // A K step is 4 sites of one colour: lane l (group q = l / 16, index n = l % 16)
 // feeds A[m = n][k = q] = g of row (o = 4 mt + n / 4, s = n % 4) at site q of
 // the group, and B[k = q][n] = v of column (i = 4 nt + n / 4, s2 = n % 4) at the
@@ -696,9 +699,8 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
         for (int nt = 0; nt < NT; ++nt) {
           const float ar = gv[mt][c].re, ai = gv[mt][c].im, br = vv[nt][c].re, bi = vv[nt][c].im;
           dr[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, dr[mt][nt], 0, 0, 0);
-          dr[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dr[mt][nt], 0, 0, 0);
-          di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, di[mt][nt], 0, 0, 0);
-          dn[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, dn[mt][nt], 0, 0, 0);
+          dn[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dn[mt][nt], 0, 0, 0);
+          di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br - bi, di[mt][nt], 0, 0, 0);
         }
   }
   // block sum in a fixed order: registers (mt, nt, part, j) of every lane
@@ -711,8 +713,8 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = ((mt * NT + nt) * 2) * 4 + j;
-        sh[wave][r][lane] = dr[mt][nt][j];
-        sh[wave][r + 4][lane] = di[mt][nt][j] - dn[mt][nt][j];
+        sh[wave][r][lane] = dr[mt][nt][j] + dn[mt][nt][j];
+        sh[wave][r + 4][lane] = di[mt][nt][j] - dr[mt][nt][j] + dn[mt][nt][j];
       }
   __syncthreads();
   for (int e = threadIdx.x; e < NREG * 64; e += blockDim.x) {
