@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=1)
     ap.add_argument("--train-batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--solution-weight", type=float, default=0.0,
+                    help="weight of the (source, solution) pair loss |K c - y|^2/|y|^2 in training "
+                         "(0: the residual loss |Mhat K c - c|^2/|c|^2 only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -102,7 +105,8 @@ def main():
         sys.exit(2)
 
     import torch
-    from qcu_amd.lattice import FP16, FP32, FP64, Lattice, LearnedPreconditioner, gaussian, random_gauge
+    from qcu_amd.lattice import (FP16, FP32, FP64, Lattice, LearnedPreconditioner, gaussian, random_gauge,
+                                 schur_solutions)
 
     inner = FP16 if args.inner == "fp16" else FP32
     U = random_gauge(L, EPS, 20241)
@@ -140,7 +144,18 @@ def main():
     net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3)
     srcs = [gaussian((vh, 4, 3), 10000 + k) for k in range(args.train_sources)]
     torch.cuda.synchronize()
-    losses, train_s = net.train(srcs, epochs=args.train_epochs, batch=args.train_batch, lr=args.lr)
+    pair_s = 0.0
+    if args.solution_weight > 0:
+        # (source, solution) pairs from the mixed-precision solver, y = Mhat^-1 c
+        t0 = time.perf_counter()
+        sols = schur_solutions(lat, srcs, tol=1e-8, prec=inner, reliable_delta=args.delta)
+        torch.cuda.synchronize()
+        pair_s = time.perf_counter() - t0
+        losses, train_s = net.train_pairs(srcs, sols, epochs=args.train_epochs, batch=args.train_batch,
+                                          lr=args.lr, solution_weight=args.solution_weight)
+        del sols
+    else:
+        losses, train_s = net.train(srcs, epochs=args.train_epochs, batch=args.train_batch, lr=args.lr)
     del srcs
     log(f"training: {train_s:.1f} s, epoch losses {losses.tolist()}")
     learned = lambda: lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta, precond=net)
@@ -207,6 +222,9 @@ def main():
         "true_res": {"learned_mixed": lres[-1].true_res, "mixed": mixed_res.true_res,
                      "plain": plain_res.true_res},
         "training": {"seconds": round(train_s, 2), "sources": args.train_sources,
+                     "loss": "residual |Mhat K c - c|^2/|c|^2" if args.solution_weight <= 0 else
+                             f"{args.solution_weight} x pair |K c - y|^2/|y|^2 + {1 - args.solution_weight:g} x residual",
+                     "pair_generation_seconds": round(pair_s, 2),
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
         "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_mfma<2, 8>)", "bound": "mfma",

@@ -224,6 +224,25 @@ def tune_load(path):
     _lib.check(_lib.load().qcuTuneLoad(str(path).encode()))
 
 
+def schur_solutions(lattice: Lattice, sources, tol=1e-8, prec=FP16, reliable_delta=0.01):
+    """y = Mhat^-1 c for odd-site sources c ([Lt][Lz][Ly][Lx/2][4][3] or [Vh][4][3]):
+    the full system M x = (0, A_o c) has the Schur right-hand side
+    bhat = A_o^-1 (A_o c) = c, so its odd half solves Mhat y = c.  A_o c is the
+    odd half of M applied to (0, c).  Returns the list of y (complex128, shape of c)."""
+    full = fermion_shape(lattice.L)
+    out = []
+    for c in sources:
+        z = torch.zeros(full, dtype=torch.complex128, device=c.device)
+        z[1] = c.reshape(full[1:])
+        bf = torch.zeros_like(z)
+        bf[1] = lattice.mat(z)[1]
+        x, res = lattice.solve(bf, tol=tol, prec=prec, reliable_delta=reliable_delta)
+        if not res.converged:
+            raise RuntimeError(f"schur_solutions: solve did not reach tol {tol}: {res}")
+        out.append(x[1].reshape(c.shape).clone())
+    return out
+
+
 class LearnedPreconditioner:
     """Gauge-covariant linear network K on one lattice (see qcuNetCreate).
 
@@ -310,6 +329,63 @@ class LearnedPreconditioner:
         _lib.check(self._lib.qcuNetTrain(self.handle, sp, n, int(epochs), int(batch), float(lr),
                                          losses.ctypes.data_as(C.POINTER(C.c_double)), C.byref(secs)))
         return losses, secs.value
+
+    def train_pairs(self, sources, solutions, epochs=1, batch=4, lr=2e-3, solution_weight=1.0):
+        """Adam on (source, solution) pairs (y = Mhat^-1 c, e.g. from schur_solutions):
+        per pair the loss w |K c - y|^2 / |y|^2 + (1 - w) |Mhat K c - c|^2 / |c|^2,
+        w = solution_weight.  The gradient of the first term comes from the
+        library's forward and backward passes (qcuNetForward / qcuNetBackward),
+        of the second from qcuNetLossGrad; Adam (beta 0.9, 0.999, eps 1e-8, on
+        the batch mean, as qcuNetTrain) runs on the host.  Returns
+        (per-epoch mean losses, seconds)."""
+        import time
+        import numpy as np
+        if len(sources) != len(solutions) or not sources:
+            raise ValueError("one solution per source")
+        w = float(solution_weight)
+        n, vh = len(sources), self.lattice.vh
+        full = (1,) + fermion_shape(self.lattice.L)
+        xin = torch.zeros(full, dtype=torch.complex128, device="cuda")
+        gout = torch.zeros_like(xin)
+        W = self.get_weights()
+        m = np.zeros(2 * W.size)
+        v = np.zeros(2 * W.size)
+        step = 0
+        losses = np.zeros(epochs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for e in range(epochs):
+            lsum = 0.0
+            for b0 in range(0, n, batch):
+                b1 = min(n, b0 + batch)
+                g = np.zeros(W.size, complex)
+                for k in range(b0, b1):
+                    c, y = sources[k], solutions[k]
+                    if w > 0:
+                        xin.zero_()
+                        xin[0, 1] = c.view(full[2:])
+                        err = self.forward(xin)[0, 1].reshape(vh, 4, 3) - y
+                        y2 = float(torch.vdot(y.flatten(), y.flatten()).real)
+                        lsum += w * float(torch.vdot(err.flatten(), err.flatten()).real) / y2
+                        gout.zero_()
+                        gout[0, 1] = (2.0 / y2) * err.view(full[2:])
+                        gs, _ = self.backward(xin, gout, want_input_grad=False)
+                        g += w * gs
+                    if w < 1:
+                        lr_, gr = self.loss_grad(c)
+                        lsum += (1 - w) * lr_
+                        g += (1 - w) * gr
+                g /= b1 - b0
+                step += 1
+                gv = np.concatenate([g.real, g.imag])
+                m = 0.9 * m + 0.1 * gv
+                v = 0.999 * v + 0.001 * gv * gv
+                d = lr * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
+                W = W - (d[:W.size] + 1j * d[W.size:])
+                self.set_weights(W)
+            losses[e] = lsum / n
+        torch.cuda.synchronize()
+        return losses, time.perf_counter() - t0
 
     def loss_grad(self, source):
         """(loss, weight gradient) of |Mhat K b - b|^2 / |b|^2 for one odd source"""

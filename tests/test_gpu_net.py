@@ -182,3 +182,24 @@ def test_replacing_the_network_recaptures_graphs(orc):
     xe, re_ = lat.solve(b, tol=1e-10, prec=FP32, precond=net2, use_graph=False)
     log(f"net1 {r1.iters} iters; net2 graph {rg.iters} / eager {re_.iters} iters")
     assert rg.converged and torch.equal(xg, xe) and rg.iters == re_.iters
+
+
+def test_schur_solutions_and_pair_training(orc):
+    """(source, solution) pairs: schur_solutions gives Mhat y = c under the oracle's
+    Mhat, and train_pairs lowers the pair loss |K c - y|^2 / |y|^2"""
+    from qcu_amd.lattice import FP64, Lattice, LearnedPreconditioner, schur_solutions
+    L, kappa, csw, tb = (8, 4, 4, 8), 0.124, 1.0, -1
+    U = orc.gauge_random(L, 0.3, 130)
+    lat = Lattice(L, kappa, csw, tb)
+    lat.load_gauge(dev(U))
+    cs = [dev(orc.gauss((orc.vol(L) // 2, 4, 3), 131 + k)) for k in range(8)]
+    ys = schur_solutions(lat, cs, tol=1e-10, prec=FP64)
+    mhat, _ = _schur_ops(orc, U, L, kappa, csw, tb)
+    worst = max(rel(mhat(host(y)), host(c)) for c, y in zip(cs, ys))
+    net = LearnedPreconditioner(lat, (1, 4, 4, 1), seed=132)
+    losses, secs = net.train_pairs(cs, ys, epochs=6, batch=4, lr=2e-4)
+    log(f"schur_solutions: max |Mhat y - c|/|c| {worst:.1e}; pair losses {[round(x, 4) for x in losses]}")
+    assert worst < 1e-8
+    assert np.isfinite(losses).all() and losses[-1] < losses[0]
+    net.close()
+    lat.close()
