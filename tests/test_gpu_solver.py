@@ -78,12 +78,18 @@ def test_solve_without_graph_is_identical(orc):
     assert torch.equal(x1, x2) and torch.equal(x1, x3)
 
 
-def test_legacy_bicgstab_entry_point(orc):
-    """ncclBistabCgQcu: kappa 0.125, periodic, stop on Re<r,r> < 1e-9 (include/define.h:67-69)"""
+@pytest.mark.parametrize("source", ["gaussian", "ones"])
+def test_legacy_bicgstab_entry_point(orc, source):
+    """ncclBistabCgQcu: kappa 0.125, periodic, stop on Re<r,r> < 1e-9 (include/define.h:67-69);
+    "ones" is the reference's own test source, every complex entry 1 + 1i
+    (test/test.nccl.bistabcg.qcu-np1.py:30-31)"""
     from qcu_amd import pyqcu
     L = (8, 8, 8, 8)
     U = orc.gauge_random(L, 0.3, 440)
-    b = orc.gauss((2, orc.vol(L) // 2, 4, 3), 441)
+    if source == "ones":
+        b = np.full((2, orc.vol(L) // 2, 4, 3), 1 + 1j)
+    else:
+        b = orc.gauss((2, orc.vol(L) // 2, 4, 3), 441)
     param = pyqcu.QcuParam()
     param.lattice_size = list(L)
     grid = pyqcu.QcuParam()
@@ -94,7 +100,7 @@ def test_legacy_bicgstab_entry_point(orc):
     xh = host(x)
     r = orc.mat_full(xh, U, L, 0.125, 0.0, 1) - b
     rr = np.vdot(r, r).real
-    log(f"ncclBistabCgQcu |b - Mx|^2 = {rr:.3e}")
+    log(f"ncclBistabCgQcu ({source} source) |b - Mx|^2 = {rr:.3e}")
     # the Schur residual is the odd part of b - Mx (Ao = 1); the even part vanishes
     assert rr < 1e-9
     assert np.array_equal(host(bd), b)
