@@ -8,7 +8,8 @@ c2  24^3x48 clover, eps 0.25, csw 1.0, kappa 0.1245, Gaussian source: FP64,
     and FP32 / FP16 inner solves at reliable-update drops 0.1, 0.01, 0.001,
     with the error of each solution against the FP64 one.
 c3a 12 spin-colour point sources on the c2 lattice in one batch (FP32 inner),
-    against 12 single solves; the 12-RHS hop against the 1-RHS hop.
+    against 12 single solves; the 12-RHS hop against the 1-RHS hop; the same
+    batch with an FP16 inner solver.
 c3b 16 Wilson lattices (4 each of 8^4, 12^4, 16^3x32, 20^3x40; eps 0.3,
     kappa 0.124, Gaussian sources) in one batch against 16 single solves.
 c4  32^3x64 clover, eps 0.2, csw 1.0, kappa 0.1248, Gaussian source: plain
@@ -109,9 +110,18 @@ def c3a():
     err = max(float(torch.linalg.norm(x - y[0]) / torch.linalg.norm(y[0])) for x, y in zip(xs, res))
     ms1 = lat.bench_hop(FP32, True, True, 50)
     ms12 = batch.bench_hop(FP32, True, 50)
+    # the same batch with an FP16 inner solver
+    lat.tune(FP16)
+    batch.tune(FP16)
+    th, (xh, rh) = timed(lambda: batch.solve(bs, tol=TOL, prec=FP16, reliable_delta=0.01), 2)
+    check(rh)
+    err16 = max(float(torch.linalg.norm(x - y[0]) / torch.linalg.norm(y[0])) for x, y in zip(xh, res))
+    ms12h = batch.bench_hop(FP16, True, 50)
     return {"batch_ms": tb * 1e3, "sequential_ms": ts * 1e3, "iters": [r.iters for r in rs],
             "max_true_res": max(r.true_res for r in rs), "max_rel_diff_vs_single": err,
-            "hop_fp32_1rhs_ms": ms1, "hop_fp32_12rhs_ms": ms12, "hop_12rhs_per_rhs_ms": ms12 / 12}
+            "hop_fp32_1rhs_ms": ms1, "hop_fp32_12rhs_ms": ms12, "hop_12rhs_per_rhs_ms": ms12 / 12,
+            "fp16": {"batch_ms": th * 1e3, "iters": [r.iters for r in rh], "max_true_res": max(r.true_res for r in rh),
+                     "max_rel_diff_vs_fp32_single": err16, "hop_fp16_12rhs_ms": ms12h}}
 
 
 def c3b():

@@ -79,3 +79,23 @@ def test_mixed_rhs_counts_and_graph_replay(orc):
         tr = orc.true_residual(host(x1[k]), host(bs[k]), U, L, 0.124, 1.0, -1)
         assert r1[k].converged and tr < 1e-10
         assert torch.equal(x1[k], x2[k]) and torch.equal(x1[k], x3[k])
+
+
+def test_twelve_sources_fp16_inner(orc):
+    """12 right-hand sides interleaved on one lattice with FP16 vectors (per-site
+    scales per right-hand side): every system reaches the oracle's true residual"""
+    from qcu_amd.lattice import FP16, Batch, Lattice, point_source
+    L, kappa, csw = (8, 8, 8, 16), 0.124, 1.0
+    U = orc.gauge_random(L, 0.3, 180)
+    lat = Lattice(L, kappa, csw, -1)
+    lat.load_gauge(dev(U))
+    bs = [point_source(L, s, c) for s in range(4) for c in range(3)]
+    batch = Batch([lat], [12])
+    xs, res = batch.solve(bs, tol=1e-10, prec=FP16, reliable_delta=0.01)
+    worst = 0.0
+    for b, x, r in zip(bs, xs, res):
+        tr = orc.true_residual(host(x), host(b), U, L, kappa, csw, -1)
+        worst = max(worst, tr)
+        assert r.converged and tr <= 1e-10
+    log(f"12 RHS FP16: iterations {[r.iters for r in res]}, worst oracle true residual {worst:.2e}")
+    batch.close()
