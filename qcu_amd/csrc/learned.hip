@@ -638,58 +638,81 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       di[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
       dn[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
-  auto spin3 = [&](const float4 *b, cf (&v)[3]) {
-    const float4 x = b[0], y = b[vh];
+  // the lane's spin (3 colours) out of its two raw float4 slots
+  auto spin3 = [&](const float4 (&r)[2], float keep, cf (&v)[3]) {
+    const float4 x = r[0], y = r[1];
     if (rodd) {
-      v[0] = mk(x.z, x.w);
-      v[1] = mk(y.x, y.y);
-      v[2] = mk(y.z, y.w);
+      v[0] = mk(keep * x.z, keep * x.w);
+      v[1] = mk(keep * y.x, keep * y.y);
+      v[2] = mk(keep * y.z, keep * y.w);
     } else {
-      v[0] = mk(x.x, x.y);
-      v[1] = mk(x.z, x.w);
-      v[2] = mk(y.x, y.y);
+      v[0] = mk(keep * x.x, keep * x.y);
+      v[1] = mk(keep * x.z, keep * x.w);
+      v[2] = mk(keep * y.x, keep * y.y);
     }
   };
-#pragma unroll 1
-  for (int grp = blockIdx.x * 4 + wave; grp < ngroups; grp += nw) {
+  // one group of 4 sites: geometry and raw operands (loaded one group ahead)
+  struct Geo {
+    int p, i;
+    float keep;
+    TermGeo tg;
+  };
+  struct Ops {
+    float4 g[MT][2], v[NT][2], l[3];
+  };
+  // lanes whose row or column channel does not exist read channel 0 and are
+  // scaled by zero, so every lane issues the same loads
+  const int o_n = n >> 2;
+  float grow[MT], gcol[NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) grow[mt] = 4 * mt + o_n < a.cout ? 1.f : 0.f;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) gcol[nt] = 4 * nt + o_n < a.cin ? 1.f : 0.f;
+  auto geo = [&](int grp) {
+    Geo r;
     int k = grp * 4 + q;
     const bool ok = k < nsite;
-    if (!ok) k = grp * 4;  // a valid site; its contribution is zeroed below
-    const int p = k >= g.vh ? 1 : 0;
-    const int i = k - p * g.vh;
-    const Site site = decode(g, i, p);
-    const float keep = ok ? 1.f : 0.f;
+    if (!ok) k = grp * 4;  // a valid site; its contribution is zeroed
+    r.p = k >= g.vh ? 1 : 0;
+    r.i = k - r.p * g.vh;
+    r.keep = ok ? 1.f : 0.f;
+    r.tg = term_geo(g, decode(g, r.i, r.p), r.i, r.p, t);
+    return r;
+  };
+  auto load = [&](Ops &x, const Geo &ge) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int o = grow[mt] != 0.f ? 4 * mt + o_n : 0;
+      const float4 *b = (const float4 *)chan_field(a.gout, o, ge.p, g.vh).p + ge.i + (size_t)rslot * vh;
+      x.g[mt][0] = b[0];
+      x.g[mt][1] = b[vh];
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int ch = gcol[nt] != 0.f ? 4 * nt + o_n : 0;
+      const float4 *b = (const float4 *)chan_field(a.act, ch, ge.tg.jp, g.vh).p + ge.tg.j + (size_t)rslot * vh;
+      x.v[nt][0] = b[0];
+      x.v[nt][1] = b[vh];
+    }
+    if (ge.tg.hop) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        x.l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
+    }
+  };
+  auto compute = [&](const Ops &x, const Geo &ge) {
     // rows: output gradient of channels 4 mt + n / 4, spin n % 4
     cf gv[MT][3];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int o = 4 * mt + (n >> 2);
-      if (o < a.cout) {
-        const SpinorPtr f = chan_field(a.gout, o, p, g.vh);
-        spin3((const float4 *)f.p + i + (size_t)rslot * vh, gv[mt]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) gv[mt][c] = mk(keep * gv[mt][c].re, keep * gv[mt][c].im);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) gv[mt][c] = mk(0.f, 0.f);
-      }
-    }
+    for (int mt = 0; mt < MT; ++mt) spin3(x.g[mt], ge.keep * grow[mt], gv[mt]);
     // columns: transported inputs of channels 4 nt + n / 4, spin n % 4
-    const TermGeo tg = term_geo(g, site, i, p, t);
     cf u[9];
-    if (tg.hop) term_link_matrix(a.U, tg, u);
+    if (ge.tg.hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
     cf vv[NT][3];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int ch = 4 * nt + (n >> 2);
-      if (ch < a.cin) {
-        const SpinorPtr f = chan_field(a.act, ch, tg.jp, g.vh);
-        spin3((const float4 *)f.p + tg.j + (size_t)rslot * vh, vv[nt]);
-        if (tg.hop) transport1(u, vv[nt]);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) vv[nt][c] = mk(0.f, 0.f);
-      }
+      spin3(x.v[nt], gcol[nt], vv[nt]);
+      if (ge.tg.hop) transport1(u, vv[nt]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -702,6 +725,34 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
           dn[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dn[mt][nt], 0, 0, 0);
           di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br - bi, di[mt][nt], 0, 0, 0);
         }
+  };
+  // groups grp, grp + nw, ...: the operands of the next group load into the
+  // other operand set while this one multiplies (the last group loads a valid
+  // dummy: itself), so no load sits behind a branch
+  int grp = blockIdx.x * 4 + wave;
+  if (grp < ngroups) {
+    Geo gA = geo(grp), gB;
+    Ops A, B;
+    load(A, gA);
+#pragma unroll 1
+    while (true) {
+      const int g1 = grp + nw;
+      const bool m1 = g1 < ngroups;
+      gB = geo(m1 ? g1 : grp);
+      load(B, gB);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(A, gA);
+      if (!m1) break;
+      grp = g1;
+      const int g2 = grp + nw;
+      const bool m2 = g2 < ngroups;
+      gA = geo(m2 ? g2 : grp);
+      load(A, gA);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(B, gB);
+      if (!m2) break;
+      grp = g2;
+    }
   }
   // block sum in a fixed order: registers (mt, nt, part, j) of every lane
   constexpr int NREG = MT * NT * 2 * 4;
