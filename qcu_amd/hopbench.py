@@ -3,6 +3,7 @@
     python -m qcu_amd.hopbench --prec 2 --clover 1 --xpay 1 --iters 50
     python -m qcu_amd.hopbench --copy-mib 1024 --iters 50   # plain copy of known size
     python -m qcu_amd.hopbench --net-layer 1 --iters 5       # learned-network layer 8 -> 8
+    python -m qcu_amd.hopbench --net-backward 5              # full forward + backward passes
 
 Runs the resident hop kernel on a 32^3x64 clover lattice (eps 0.2, csw 1.0,
 kappa 0.1248) and prints the HIP-event time per launch and the algorithmic
@@ -29,6 +30,7 @@ def main():
     ap.add_argument("--remap", type=int, default=-1)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--net-layer", type=int, default=-1)
+    ap.add_argument("--net-backward", type=int, default=0, help="time N full backward passes of the network")
     args = ap.parse_args()
 
     import torch
@@ -56,6 +58,22 @@ def main():
     lat.load_gauge(U)
     del U
     torch.cuda.empty_cache()
+    if args.net_backward > 0:
+        import time
+        from .lattice import LearnedPreconditioner, fermion_shape, gaussian
+        net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3, init_scale=0.3)
+        x = gaussian((1,) + fermion_shape(L), 5)
+        gout = gaussian((1,) + fermion_shape(L), 6)
+        net.backward(x, gout)
+        ts = []
+        for _ in range(args.net_backward):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            net.backward(x, gout)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        print(json.dumps({"L": L, "net_backward_ms_median": sorted(ts)[len(ts) // 2]}))
+        return
     if args.net_layer >= 0:
         from .lattice import LearnedPreconditioner, fermion_shape, gaussian
         net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3)
