@@ -13,7 +13,11 @@
  *   - boundaries are periodic in all four directions (the reference kernel
  *     wraps periodically, include/define.h:127-134);
  *   - calls are synchronous on return and leave every input bit-identical
- *     (the reference transposes gauge and fermion_in in place and back).
+ *     (the reference transposes gauge and fermion_in in place and back);
+ *   - the converted links and the clover term stay resident while the same
+ *     gauge pointer holds the same contents (a 64-bit hash of the field,
+ *     computed on the device at every call, decides; the reference rebuilds
+ *     them at every call).
  * Errors never abort the process and never throw: the void entry points record
  * the failure, readable with qcuGetLastError()/qcuGetLastErrorString().
  *
