@@ -17,9 +17,11 @@
 // Kernels:
 // * layer forward on the matrix cores (k_net_layer_mfma: the channel and spin
 //   mixing as real FP32 MFMA products, one colour transport per lane and step)
-//   for layers with several output channels, and on the vector units
-//   (k_net_layer, one thread per site and group of output channels) for the
-//   single-output layer and as a cross-check;
+//   for layers with several output channels; a single-output layer mixes its
+//   input channels per term on the matrix cores (k_net_mix) and transports
+//   one mixed field per term (k_net_gather); the vector-unit kernel
+//   (k_net_layer, one thread per site and group of output channels) is the
+//   cross-check;
 // * the data backward is the forward kernel with conjugate-transposed weights
 //   and swapped hop directions;
 // * the weight gradient is a deterministic two-stage reduction of
@@ -530,6 +532,163 @@ __global__ void __launch_bounds__(256) k_net_layer_mfma(NetLayerArgs a, const fl
   net_layer_mfma_body<MT, CI>(a, Apk);
 }
 
+// ---- single-output layer in two passes --------------------------------------
+// With one output channel the matrix-core layer fills a quarter of its tiles
+// and still transports every input channel.  Mixing and transport commute, so
+// the layer is split:
+//   pass 1 (k_net_mix, matrix cores) mixes the input channels once per term at
+//     every input site, Y_t(j) = sum_i W[t][0][i] in_i(j): per colour a complex
+//     GEMM with m = (term, spin), k = (spin s2) per input channel, n = site, as
+//     three real products; two M tiles hold the 8 hop terms, a third (rows 0-3
+//     used) the site term;
+//   pass 2 (k_net_gather, vector units) transports and sums them at the output
+//     sites, out(x) = sum_{t<8} M_t(x) Y_t(neighbour_t(x)) + Y_8(x),
+// one transport per term instead of one per term and input channel.  The term
+// fields Y_t use the channel layout: term t, parity q at slot (t, q).
+struct NetMixArgs {
+  SpinorPtr in;     // cin channel fields
+  SpinorPtr mix;    // 9 term fields
+  const float2 *W;  // the layer's weights W[t][0][i][4x4]
+  int vh, cin;
+  int out_par;      // the output parity of a one-parity layer (-1: both parities)
+  const SysState *st;
+};
+
+// tiles T0 .. T0 + NT - 1 of input parity q for the 16 sites from base
+template <int CI, int T0, int NT>
+QCU_D void net_mix_body(const NetMixArgs &a, int q, int base) {
+  const int lane = threadIdx.x & 63, gq = lane >> 4, ml = lane & 15;
+  const int vh = a.vh;
+  int i = base + ml;
+  const bool ok = i < vh;
+  if (!ok) i = base;
+  // the lane's spin gq of site i: floats 6 gq .. 6 gq + 5 of the 24
+  const unsigned vh16 = (unsigned)vh * 16u;
+  unsigned off[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int f = 6 * gq + 2 * k;
+    off[k] = (unsigned)(f >> 2) * vh16 + (unsigned)(f & 3) * 4u + (unsigned)i * 16u;
+  }
+  f4v p1[3][NT], p2[3][NT], p3[3][NT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int e = 0; e < NT; ++e) {
+      p1[c][e] = (f4v){0.f, 0.f, 0.f, 0.f};
+      p2[c][e] = (f4v){0.f, 0.f, 0.f, 0.f};
+      p3[c][e] = (f4v){0.f, 0.f, 0.f, 0.f};
+    }
+  const int cin = CI > 0 ? CI : a.cin;
+  const size_t field = (size_t)vh * 96;
+  const char *fb = (const char *)a.in.p + (size_t)q * field;
+  // A[m = ml][k = gq] of tile et: term 4 et + ml / 4 (tile 2: the site term in
+  // rows 0-3, zero below), spin ml % 4, input spin gq
+  auto step = [&](int ci) {
+    const char *cb = fb + (size_t)ci * 2 * field;
+    float2 s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = ld_at<float2>(cb, 0, off[k]);
+    float w[NT][3];
+#pragma unroll
+    for (int e = 0; e < NT; ++e) {
+      const int et = T0 + e;
+      const int t = et < 2 ? 4 * et + (ml >> 2) : 8;
+      float2 wv = a.W[((size_t)t * cin + ci) * 16 + (ml & 3) * 4 + gq];
+      if (et == 2 && ml >= 4) wv = make_float2(0.f, 0.f);
+      w[e][0] = wv.x;
+      w[e][1] = wv.y;
+      w[e][2] = wv.x + wv.y;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float br = s[c].x, bi = s[c].y, bs = br + bi;
+#pragma unroll
+      for (int e = 0; e < NT; ++e) {
+        p1[c][e] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[e][0], br, p1[c][e], 0, 0, 0);
+        p2[c][e] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[e][1], bi, p2[c][e], 0, 0, 0);
+        p3[c][e] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[e][2], bs, p3[c][e], 0, 0, 0);
+      }
+    }
+  };
+  if constexpr (CI > 0) {
+#pragma unroll
+    for (int ci = 0; ci < CI; ++ci) step(ci);
+  } else {
+#pragma unroll 1
+    for (int ci = 0; ci < cin; ++ci) step(ci);
+  }
+  if (!ok) return;
+  // lane group gq holds rows 4 gq .. 4 gq + 3: spins 0..3 of term 4 et + gq
+#pragma unroll
+  for (int e = 0; e < NT; ++e) {
+    const int et = T0 + e;
+    if (et == 2 && gq != 0) continue;
+    const int t = et < 2 ? 4 * et + gq : 8;
+    cf out[12];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float x = p1[c][e][j], y = p2[c][e][j];
+        out[j * 3 + c] = mk(x - y, p3[c][e][j] - x - y);
+      }
+    SpinorIO<P32>::store(chan_field(a.mix, t, q, vh), i, out);
+  }
+}
+
+// grid: x = 16-site chunks (one per wave), y = input parity q.  BOTH: every
+// term on both parities (both output parities); otherwise the hop terms on
+// the parity the output parity's hops read and the site term on the output
+// parity.
+template <int CI, bool BOTH>
+__global__ void __launch_bounds__(256) k_net_mix(NetMixArgs a) {
+  if (a.st && a.st->done) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int base = ((int)blockIdx.x * (int)(blockDim.x >> 6) + wave) * 16;
+  if (base >= a.vh) return;  // uniform over the wave
+  const int q = blockIdx.y;
+  if constexpr (BOTH) {
+    net_mix_body<CI, 0, 3>(a, q, base);
+  } else {
+    if (q == a.out_par) net_mix_body<CI, 2, 1>(a, q, base);
+    else net_mix_body<CI, 0, 2>(a, q, base);
+  }
+}
+
+struct NetGatherArgs {
+  SpinorPtr mix;  // the 9 term fields of k_net_mix
+  SpinorPtr out;  // the output channel
+  LinkPtr U;      // FP32 links
+  Geom g;
+  int parity0;    // first output parity (the grid's y index adds to it)
+  const SysState *st;
+};
+
+// one thread per output site: the site term plus the 8 transported hop terms
+__global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
+  if (a.st && a.st->done) return;
+  const Geom g = a.g;
+  const int i = remap_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
+  const int p = a.parity0 + blockIdx.y;
+  if (i >= g.vh) return;
+  const Site site = decode(g, i, p);
+  cf acc[12];
+  SpinorIO<P32>::load(chan_field(a.mix, 8, p, g.vh), i, acc);
+#pragma unroll 2
+  for (int t = 0; t < 8; ++t) {
+    const TermGeo tg = term_geo(g, site, i, p, t);
+    cf m[9];
+    term_link_matrix(a.U, tg, m);
+    cf v[12];
+    SpinorIO<P32>::load(chan_field(a.mix, t, tg.jp, g.vh), tg.j, v);
+    transport_spins(m, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = acc[k] + v[k];
+  }
+  SpinorIO<P32>::store(chan_field(a.out, 0, p, g.vh), i, acc);
+}
+
 // weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
 //   sum over sites of both parities and colours of g_o(x)[s][c] conj((T_t a_ci)(x)[s2][c])
 struct NetWgradArgs {
@@ -1000,6 +1159,7 @@ struct Net : Precond {
   int co_max = 4;   // output channels per thread of the vector layer kernel (1, 2, 3, 4 or 8)
   bool use_mfma = true;  // matrix-core layer kernel (QCU_NET_KERNEL=valu selects the vector one)
   DevBuf Apk;       // A operands of the matrix-core kernel for one layer
+  DevBuf mixb;      // term fields of the two-pass single-output layer (9 fields)
   int block = 128;  // threads per block of the layer kernel
   int tile = 8;     // rows per tile side of the matrix-core wave order (0: site order)
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
@@ -1097,6 +1257,11 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 2 * kApkParts * 64 * 4));
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
+  // a layer with one output and several input channels (the last layer, and
+  // the input gradient of the first) runs in two passes through mixb
+  int cbig = 1;
+  for (int k = 0; k <= nl; ++k) cbig = std::max(cbig, chan[k]);
+  if (use_mfma && cbig > 1) QCU_TRY(mixb.alloc(9 * field_bytes()));
   // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
   if (const char *e = getenv("QCU_NET_SHAPE")) {
     int c = 0, b = 0;
@@ -1122,8 +1287,42 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.parity0 = parity0;
   a.zero_par = zero_par;
   a.st = st;
-  // a single output channel fills a quarter of the matrix-core tiles while the
-  // transports cost the same, so that layer runs on the vector units
+  // one output channel: the input channels are mixed per term on the matrix
+  // cores at the input sites, then transported once per term
+  if (use_mfma && cout == 1 && cin > 1 && zero_par < 0) {
+    QCU_CHECK(mixb.p && (nparity == 1 || parity0 == 0), QCU_ERR_STATE, "net: two-pass layer set-up");
+    const int vh = lat->g.vh;
+    NetMixArgs m;
+    m.in = in;
+    m.mix.p = mixb.p;
+    m.mix.norm = nullptr;
+    m.mix.stride = vh;
+    m.W = Wl;
+    m.vh = vh;
+    m.cin = cin;
+    m.out_par = nparity == 2 ? -1 : parity0;
+    m.st = st;
+    const dim3 g1((unsigned)ndiv(vh, 64), 2);  // 4 waves of 16 sites per block
+    const bool both = nparity == 2;
+    if (cin == 8) {
+      if (both) hipLaunchKernelGGL((k_net_mix<8, true>), g1, dim3(256), 0, s, m);
+      else hipLaunchKernelGGL((k_net_mix<8, false>), g1, dim3(256), 0, s, m);
+    } else {
+      if (both) hipLaunchKernelGGL((k_net_mix<0, true>), g1, dim3(256), 0, s, m);
+      else hipLaunchKernelGGL((k_net_mix<0, false>), g1, dim3(256), 0, s, m);
+    }
+    QCU_HIP(hipGetLastError());
+    NetGatherArgs ga;
+    ga.mix = m.mix;
+    ga.out = out;
+    ga.U = a.U;
+    ga.g = lat->g;
+    ga.parity0 = parity0;
+    ga.st = st;
+    hipLaunchKernelGGL(k_net_gather, dim3((unsigned)ndiv(vh, 256), nparity), dim3(256), 0, s, ga);
+    QCU_HIP(hipGetLastError());
+    return 0;
+  }
   if (use_mfma && cout > 1) {
     const int mtiles = (cout + 3) / 4;
     const int total = 9 * cin * mtiles * kApkParts * 64;
@@ -1600,10 +1799,12 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
     // (16 complex multiply-adds each) + 8 hops x cin transports of 4 spins
     // (9 complex multiply-adds each).  A complex multiply-add is 8 real flops,
     // except in the matrix-core mixing, which uses three real products per
-    // complex product (6 flops)
-    const bool mfma = net.use_mfma && co > 1;
+    // complex product (6 flops).  The two-pass single-output layer transports
+    // one mixed field per hop term.
+    const bool two_pass = net.use_mfma && co == 1 && ci > 1;
+    const bool mfma = net.use_mfma && (co > 1 || two_pass);
     const double mix = (mfma ? 6.0 : 8.0) * 9.0 * co * ci * 16 * 3;
-    const double per_site = mix + 8.0 * (8.0 * ci * 4 * 9);
+    const double per_site = mix + 8.0 * (8.0 * (two_pass ? 1 : ci) * 4 * 9);
     *flops = per_site * 2.0 * net.lat->g.vh;
   }
   return 0;
