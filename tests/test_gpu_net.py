@@ -35,10 +35,11 @@ def _setup(orc, L, chan, tb, seed, csw=0.0, kappa=0.124, eps=0.4):
 
 # Lx / 2 a multiple of 16 selects the tiled wave order of the matrix-core layer
 # kernel (tiles of 8 x 6 and 2 x 4 rows here); the smaller lattices run it in
-# site order
+# site order.  (6, 2, 2, 2) has 24 sites per parity: the matrix-core kernels
+# (16 sites per wave) end on a partial wave there.
 @pytest.mark.parametrize("L,chan,tb", [((8, 4, 4, 8), (1, 8, 8, 8, 1), -1), ((4, 6, 4, 4), (1, 2, 3, 1), 1),
                                        ((6, 4, 6, 2), (1, 1), -1), ((32, 8, 6, 4), (1, 8, 5, 8, 1), -1),
-                                       ((32, 2, 4, 2), (1, 3, 1), 1)])
+                                       ((32, 2, 4, 2), (1, 3, 1), 1), ((6, 2, 2, 2), (1, 8, 3, 1), 1)])
 def test_net_forward_and_backward_vs_oracle(orc, L, chan, tb):
     U, lat, net, W = _setup(orc, L, chan, tb, 100)
     assert net.nweights == orc.net_weight_count(chan)
@@ -58,7 +59,7 @@ def test_net_forward_and_backward_vs_oracle(orc, L, chan, tb):
     assert e_gw < 1e-4
 
 
-@pytest.mark.parametrize("L", [(8, 8, 4, 4), (32, 4, 4, 4)])
+@pytest.mark.parametrize("L", [(8, 8, 4, 4), (32, 4, 4, 4), (6, 2, 2, 2)])
 def test_net_apply_odd_is_odd_part_of_forward(orc, L):
     chan = (1, 4, 4, 1)
     U, lat, net, W = _setup(orc, L, chan, -1, 110)
