@@ -153,6 +153,28 @@ def test_trained_preconditioner_reduces_iterations(orc):
     assert r1.iters < r0.iters
 
 
+@pytest.mark.parametrize("fill", [0.0, np.nan])
+def test_broken_preconditioner_falls_back_to_plain(orc, fill):
+    """a network whose output is zero (or NaN) breaks the preconditioned
+    BiCGSTAB down at its first step (<r~, Mhat K p> is 0, or not finite); after
+    the bounded restarts the solve continues without the network from the
+    current solution and still meets |b - M x|/|b| <= 1e-10"""
+    from qcu_amd.lattice import FP16, Lattice, LearnedPreconditioner
+    L, kappa, csw, tb = (8, 4, 4, 8), 0.124, 1.0, -1
+    U = orc.gauge_random(L, 0.3, 145)
+    lat = Lattice(L, kappa, csw, tb)
+    lat.load_gauge(dev(U))
+    net = LearnedPreconditioner(lat, (1, 4, 1), seed=146)
+    net.set_weights(np.full(net.nweights, complex(fill, fill)))
+    b = orc.gauss((2, orc.vol(L) // 2, 4, 3), 147)
+    x, r = lat.solve(dev(b), tol=1e-10, prec=FP16, reliable_delta=0.01, precond=net)
+    tr = orc.true_residual(host(x), b, U, L, kappa, csw, tb)
+    log(f"weights {fill}: {r.iters} iters, {r.restarts} restarts, fallback {r.fallback}, oracle true residual {tr:.1e}")
+    assert r.fallback and r.converged and tr <= 1e-10
+    net.close()
+    lat.close()
+
+
 def test_vector_layer_kernel_matches_oracle(orc, monkeypatch):
     """the vector-FMA layer kernel (kept as a cross-check of the matrix-core one)"""
     monkeypatch.setenv("QCU_NET_KERNEL", "valu")
