@@ -1,7 +1,7 @@
 """GPU parity of batched solves: many right-hand sides sharing one lattice's
 link loads, and several lattices of different shapes in one launch per kernel.
 
-Every system is judged by its FP64 true residual recomputed by the host
+Every system is checked by its FP64 true residual recomputed by the host
 oracle, and compared with the single-system solve of the same system.
 """
 import numpy as np

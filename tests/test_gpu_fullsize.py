@@ -6,7 +6,7 @@ the tiled wave order of the network kernel take their real sizes:
 * the FP64 clover hop against the oracle's (OpenMP, FP64) to 1e-12;
 * the matrix-core network (tiled wave order) against the vector-FMA network,
   itself pinned against the oracle on small lattices;
-* a mixed-precision FP16 solve judged by the oracle's FP64 true residual.
+* a mixed-precision FP16 solve checked by the oracle's FP64 true residual.
 The oracle takes seconds per call at this size (gauge generation, one hop, one
 residual).
 """

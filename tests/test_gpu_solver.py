@@ -1,6 +1,6 @@
 """GPU parity of the even-odd BiCGSTAB solvers (FP64 and mixed precision).
 
-Every solve is judged by the FP64 true residual |b - M x| / |b| recomputed by
+Every solve is checked by the FP64 true residual |b - M x| / |b| recomputed by
 the host oracle; solutions are compared with the oracle's own BiCGSTAB
 solution.  The stopping target is 1e-10 (the true residual required of every
 solve); the solution tolerance follows from the condition number.
