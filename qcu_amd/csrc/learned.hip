@@ -776,16 +776,24 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // same site; it ends holding D rows (o = 4 mt + q, s = j), column n.  Each wave
 // sweeps groups of 4 sites; the 4 waves of a block are summed through LDS and
 // the block sums are added in a fixed order by k_net_wgrad_fin.
-constexpr int kWgBlocks = 256;  // blocks per term
+//
+// The grid is one row of 9 x kWgBlocks blocks.  The 9 blocks (one per term)
+// of a site block sb are 8 block indices apart inside a window of 72: blocks
+// go to the XCDs in turn, so all 9 run at about the same time on one XCD and
+// read the output gradient and the input field of their sites from its L2,
+// instead of from HBM once per term.
+constexpr int kWgBlocks = 256;  // blocks per term (a multiple of 8)
 
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
   const Geom g = a.g;
-  const int t = blockIdx.y;
+  const int b = blockIdx.x;
+  const int t = (b >> 3) % 9, sb = (b / 72) * 8 + (b & 7);
+  const int nsb = gridDim.x / 9;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, n = lane & 15;
   const int nsite = 2 * g.vh, ngroups = (nsite + 3) / 4;
-  const int nw = gridDim.x * 4;
+  const int nw = nsb * 4;
   const size_t vh = (size_t)g.vh;
   const int rs = n & 3, rslot = (3 * rs) >> 1, rodd = (3 * rs) & 1;  // spin of this lane's row / column
   f4v dr[MT][NT], di[MT][NT], dn[MT][NT];
@@ -888,7 +896,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   // groups grp, grp + nw, ...: the operands of the next group load into the
   // other operand set while this one multiplies (the last group loads a valid
   // dummy: itself), so no load sits behind a branch
-  int grp = blockIdx.x * 4 + wave;
+  int grp = sb * 4 + wave;
   if (grp < ngroups) {
     Geo gA = geo(grp), gB;
     Ops A, B;
@@ -930,7 +938,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   for (int e = threadIdx.x; e < NREG * 64; e += blockDim.x) {
     const int r = e >> 6, l = e & 63;
     const float v = (sh[0][r][l] + sh[1][r][l]) + (sh[2][r][l] + sh[3][r][l]);
-    part[((size_t)(t * gridDim.x + blockIdx.x) * NREG + r) * 64 + l] = v;
+    part[((size_t)(t * nsb + sb) * NREG + r) * 64 + l] = v;
   }
 }
 
@@ -1415,7 +1423,8 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
     const int total = 9 * cout * cin * 16;
     if (use_mfma) {
       const int mt = (cout + 3) / 4, nt = (cin + 3) / 4;
-      const dim3 grid(kWgBlocks, 9);
+      static_assert(kWgBlocks % 8 == 0, "term blocks of a site block share an XCD");
+      const dim3 grid(9 * kWgBlocks);
       float *pp = (float *)wpart.p;
       float2 *dst = (float2 *)gW.p + woff[k];
       const dim3 fg((unsigned)ndiv(total, 256));
