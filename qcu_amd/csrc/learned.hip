@@ -781,15 +781,17 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // of a site block sb are 8 block indices apart inside a window of 72: blocks
 // go to the XCDs in turn, so all 9 run at about the same time on one XCD and
 // read the output gradient and the input field of their sites from its L2,
-// instead of from HBM once per term.
+// instead of from HBM once per term.  XCD k takes the site blocks
+// k nsb / 8 .. (k + 1) nsb / 8 - 1, so at each step of the waves' sweep it
+// works on one contiguous stretch of sites (their y neighbours included).
 constexpr int kWgBlocks = 256;  // blocks per term (a multiple of 8)
 
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
   const Geom g = a.g;
   const int b = blockIdx.x;
-  const int t = (b >> 3) % 9, sb = (b / 72) * 8 + (b & 7);
   const int nsb = gridDim.x / 9;
+  const int t = (b >> 3) % 9, sb = (b & 7) * (nsb >> 3) + b / 72;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, n = lane & 15;
   const int nsite = 2 * g.vh, ngroups = (nsite + 3) / 4;
