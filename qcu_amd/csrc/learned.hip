@@ -784,7 +784,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // instead of from HBM once per term.  XCD k takes the site blocks
 // k nsb / 8 .. (k + 1) nsb / 8 - 1, so at each step of the waves' sweep it
 // works on one contiguous stretch of sites (their y neighbours included).
-constexpr int kWgBlocks = 256;  // blocks per term (a multiple of 8)
+constexpr int kWgBlocks = 512;  // blocks per term (a multiple of 8)
 
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
@@ -958,13 +958,30 @@ __global__ void k_net_wgrad_fin(const float *part, int nblk, int cin, int cout, 
   const int mt = o >> 2, nt = ci >> 2;
   const int lane = (o & 3) * 16 + (ci & 3) * 4 + s2;
   const int r = ((mt * NT + nt) * 2) * 4 + s;
-  double re = 0, im = 0;
-  for (int b = 0; b < nblk; ++b) {
-    const float *src = part + ((size_t)(t * nblk + b) * NREG + r) * 64 + lane;
-    re += src[0];
-    im += src[4 * 64];
+  // 8 interleaved FP64 sums (block b into sum b % 8) keep 16 loads in flight;
+  // they are combined in a fixed order, so the result is the same every run
+  constexpr int kAcc = 8;
+  double re[kAcc] = {}, im[kAcc] = {};
+  const float *src = part + ((size_t)t * nblk * NREG + r) * 64 + lane;
+  const size_t bstride = (size_t)NREG * 64;
+  int b = 0;
+  for (; b + kAcc <= nblk; b += kAcc)
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) {
+      re[k] += src[(b + k) * bstride];
+      im[k] += src[(b + k) * bstride + 4 * 64];
+    }
+  for (; b < nblk; ++b) {
+    re[b % kAcc] += src[b * bstride];
+    im[b % kAcc] += src[b * bstride + 4 * 64];
   }
-  gW[idx] = make_float2((float)re, (float)im);
+  double sr = 0, si = 0;
+#pragma unroll
+  for (int k = 0; k < kAcc; ++k) {
+    sr += re[k];
+    si += im[k];
+  }
+  gW[idx] = make_float2((float)sr, (float)si);
 }
 
 // second stage: gW[t][o][ci][16] = sum over blocks (FP64 accumulation)
