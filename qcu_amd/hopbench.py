@@ -4,12 +4,15 @@
     python -m qcu_amd.hopbench --copy-mib 1024 --iters 50   # plain copy of known size
     python -m qcu_amd.hopbench --net-layer 1 --iters 5       # learned-network layer 8 -> 8
     python -m qcu_amd.hopbench --net-backward 5              # full forward + backward passes
+    python -m qcu_amd.hopbench --net-apply 20                # preconditioner applications
 
 Runs the resident hop kernel on a 32^3x64 clover lattice (eps 0.2, csw 1.0,
 kappa 0.1248) and prints the HIP-event time per launch and the algorithmic
 GB/s.  The copy mode launches a device-to-device copy of a known byte count,
 used to calibrate the FETCH_SIZE / WRITE_SIZE counters.  The net-layer mode
 times one forward layer (both parities) of the 1-8-8-8-1 channel network.
+The net-apply mode times the network applied as the preconditioner (an odd
+field in, the odd part of the output kept), as in the learned solve.
 """
 import argparse
 import json
@@ -31,6 +34,7 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--net-layer", type=int, default=-1)
     ap.add_argument("--net-backward", type=int, default=0, help="time N full backward passes of the network")
+    ap.add_argument("--net-apply", type=int, default=0, help="time N preconditioner applications of the network")
     args = ap.parse_args()
 
     import torch
@@ -58,6 +62,22 @@ def main():
     lat.load_gauge(U)
     del U
     torch.cuda.empty_cache()
+    if args.net_apply > 0:
+        import time
+        from .lattice import LearnedPreconditioner, gaussian
+        net = LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=3, init_scale=0.3)
+        x = gaussian((lat.vh, 4, 3), 7)
+        for _ in range(3):
+            net.apply_odd(x)
+        ts = []
+        for _ in range(args.net_apply):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            net.apply_odd(x)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        print(json.dumps({"L": L, "net_apply_ms_median": sorted(ts)[len(ts) // 2]}))
+        return
     if args.net_backward > 0:
         import time
         from .lattice import LearnedPreconditioner, fermion_shape, gaussian
