@@ -51,8 +51,15 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.iters
+        # a write-only stream of the same size
+        e0.record()
+        for _ in range(args.iters):
+            b.fill_(1)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_fill = e0.elapsed_time(e1) / args.iters
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "ms": ms,
-                          "GBps_rw": 2 * n / ms / 1e6}))
+                          "GBps_rw": 2 * n / ms / 1e6, "fill_ms": ms_fill, "fill_GBps": n / ms_fill / 1e6}))
         return
 
     from .lattice import Lattice, random_gauge
