@@ -51,7 +51,9 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.iters
-        # a write-only stream of the same size
+        # a write-only stream of the same size (one untimed call loads the kernel)
+        b.fill_(1)
+        torch.cuda.synchronize()
         e0.record()
         for _ in range(args.iters):
             b.fill_(1)
