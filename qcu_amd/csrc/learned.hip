@@ -1203,7 +1203,8 @@ struct Net : Precond {
             const SysState *st, hipStream_t s, int zero_par = -1);
   // zero_par0: parity on which the network input is zero (-1: none)
   int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
-  int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in);
+  // gout_zero_par: parity on which gout_last is zero (-1: none)
+  int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1);
   int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
   unsigned long long id = 0;
   unsigned long long uid() const override { return id; }
@@ -1418,7 +1419,7 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
 }
 
 // gout_last: gradient w.r.t. act[nl]; accumulates nothing, writes gW (this sample)
-int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr grad_in) {
+int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par) {
   // conjugate-transposed weights for the data gradient
   for (int k = 0; k < nl; ++k) {
     const int total = 9 * chan[k] * chan[k + 1] * 16;
@@ -1477,7 +1478,8 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
       gnext.norm = nullptr;
       gnext.stride = lat->g.vh;
     }
-    QCU_TRY(layer(gcur, gnext, (const float2 *)Wt.p + woff[k], cout, cin, 0, 2, nullptr, s));
+    QCU_TRY(layer(gcur, gnext, (const float2 *)Wt.p + woff[k], cout, cin, 0, 2, nullptr, s,
+                  k == nl - 1 ? gout_zero_par : -1));
     gcur = gnext;
     ping ^= 1;
   }
@@ -1562,7 +1564,9 @@ struct NetTrainer {
     // y = K b / |b|
     hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), odd(0), vh, nullptr);
     QCU_HIP(hipGetLastError());
-    QCU_TRY(net.forward(nullptr, s, false, 0));
+    // only the odd part of the output enters the loss: the last layer runs on
+    // the odd sites only, as in the preconditioner
+    QCU_TRY(net.forward(nullptr, s, true, 0));
     hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, odd(1), net.act_ptr(net.nl), vh, nullptr);
     QCU_HIP(hipGetLastError());
     QCU_TRY(mhat32(lat, odd(2), odd(1), odd(3), s));
@@ -1584,9 +1588,10 @@ struct NetTrainer {
     hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, gl, odd(2), vh, nullptr);
     QCU_HIP(hipGetLastError());
     // the backward pass ping-pongs through gbuf[0] and gbuf[1]; the last layer's
-    // output gradient lives in gbuf[1] and is consumed first
+    // output gradient lives in gbuf[1] and is consumed first.  It is zero on the
+    // even sites.
     SpinorPtr none{};
-    QCU_TRY(net.backward(gl, s, false, none));
+    QCU_TRY(net.backward(gl, s, false, none, 0));
     (void)gsum;
     return 0;
   }
