@@ -698,6 +698,8 @@ struct NetWgradArgs {
   Geom g;
   int cin, cout;
   float *partials;  // [combo][block][CO * 32]
+  int act_zero;     // parity on which the input is known to be zero (-1: none)
+  int gout_zero;    // parity on which the output gradient is known to be zero (-1: none)
 };
 
 template <int CO>
@@ -794,7 +796,17 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   const int t = (b >> 3) % 9, sb = (b & 7) * (nsb >> 3) + b / 72;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, n = lane & 15;
-  const int nsite = 2 * g.vh, ngroups = (nsite + 3) / 4;
+  // output parities this term needs: a nonzero output gradient there, and an
+  // input (parity 1 - p for a hop, p for the site term) not known to be zero;
+  // the sites of the others contribute exact zeros and are skipped
+  int pmask = 3;
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp) {
+    const int pin = t < 8 ? 1 - pp : pp;
+    if (pp == a.gout_zero || pin == a.act_zero) pmask &= ~(1 << pp);
+  }
+  const int site0 = (pmask & 1) ? 0 : g.vh;
+  const int nsite = ((pmask & 1) + (pmask >> 1)) * g.vh, ngroups = (nsite + 3) / 4;
   const int nw = nsb * 4;
   const size_t vh = (size_t)g.vh;
   const int rs = n & 3, rslot = (3 * rs) >> 1, rodd = (3 * rs) & 1;  // spin of this lane's row / column
@@ -842,6 +854,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     int k = grp * 4 + q;
     const bool ok = k < nsite;
     if (!ok) k = grp * 4;  // a valid site; its contribution is zeroed
+    k += site0;
     r.p = k >= g.vh ? 1 : 0;
     r.i = k - r.p * g.vh;
     r.keep = ok ? 1.f : 0.f;
@@ -1203,8 +1216,10 @@ struct Net : Precond {
             const SysState *st, hipStream_t s, int zero_par = -1);
   // zero_par0: parity on which the network input is zero (-1: none)
   int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
-  // gout_zero_par: parity on which gout_last is zero (-1: none)
-  int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1);
+  // gout_zero_par: parity on which gout_last is zero; act0_zero_par: parity on
+  // which the network input act[0] is zero (-1: none)
+  int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1,
+               int act0_zero_par = -1);
   int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
   unsigned long long id = 0;
   unsigned long long uid() const override { return id; }
@@ -1419,7 +1434,8 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
 }
 
 // gout_last: gradient w.r.t. act[nl]; accumulates nothing, writes gW (this sample)
-int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par) {
+int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par,
+                  int act0_zero_par) {
   // conjugate-transposed weights for the data gradient
   for (int k = 0; k < nl; ++k) {
     const int total = 9 * chan[k] * chan[k + 1] * 16;
@@ -1440,6 +1456,8 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
     a.cin = cin;
     a.cout = cout;
     a.partials = (float *)wpart.p;
+    a.act_zero = k == 0 ? act0_zero_par : -1;
+    a.gout_zero = k == nl - 1 ? gout_zero_par : -1;
     const int total = 9 * cout * cin * 16;
     if (use_mfma) {
       const int mt = (cout + 3) / 4, nt = (cin + 3) / 4;
@@ -1589,9 +1607,9 @@ struct NetTrainer {
     QCU_HIP(hipGetLastError());
     // the backward pass ping-pongs through gbuf[0] and gbuf[1]; the last layer's
     // output gradient lives in gbuf[1] and is consumed first.  It is zero on the
-    // even sites.
+    // even sites, and so is the network input.
     SpinorPtr none{};
-    QCU_TRY(net.backward(gl, s, false, none, 0));
+    QCU_TRY(net.backward(gl, s, false, none, 0, 0));
     (void)gsum;
     return 0;
   }
