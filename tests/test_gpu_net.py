@@ -95,9 +95,14 @@ def _schur_ops(orc, U, L, kappa, csw, tb):
     return mhat, mhat_dag
 
 
-@pytest.mark.parametrize("csw", [0.0, 1.0])
-def test_training_gradient_vs_oracle(orc, csw):
-    L, chan, tb, kappa = (8, 4, 4, 8), (1, 4, 4, 1), -1, 0.124
+# In training the network input and the output gradient are zero on the even
+# sites, and the first and last layers' weight gradients sweep only the
+# parities that can contribute; (6, 2, 2, 2) (24 sites per parity) ends those
+# sweeps on a partial 4-site group and a partial 16-site wave.
+@pytest.mark.parametrize("L,chan,csw", [((8, 4, 4, 8), (1, 4, 4, 1), 0.0), ((8, 4, 4, 8), (1, 4, 4, 1), 1.0),
+                                        ((6, 2, 2, 2), (1, 8, 3, 1), 1.0)])
+def test_training_gradient_vs_oracle(orc, L, chan, csw):
+    tb, kappa = -1, 0.124
     U, lat, net, W = _setup(orc, L, chan, tb, 120, csw=csw, kappa=kappa)
     net.set_weights(W * 0.2)  # a smaller net keeps Mhat K b near b
     W = net.get_weights()
@@ -120,7 +125,8 @@ def test_training_gradient_vs_oracle(orc, csw):
     _, gW_ref = orc.net_backward(gy, W, chan, U, L, acts, tb)
     loss, gW = net.loss_grad(dev(b))
     e_l, e_g = abs(loss - loss_ref) / loss_ref, rel(gW, gW_ref)
-    log(f"csw={csw}: loss {loss:.6f} (oracle {loss_ref:.6f}, rel {e_l:.1e}), gradient rel {e_g:.1e}")
+    log(f"L={L} chan={chan} csw={csw}: loss {loss:.6f} (oracle {loss_ref:.6f}, rel {e_l:.1e}), "
+        f"gradient rel {e_g:.1e}")
     # FP32 operator and network; the loss is a difference of O(1) terms
     assert e_l < 1e-4 and e_g < 1e-3
 
