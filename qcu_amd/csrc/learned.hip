@@ -786,7 +786,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // instead of from HBM once per term.  XCD k takes the site blocks
 // k nsb / 8 .. (k + 1) nsb / 8 - 1, so at each step of the waves' sweep it
 // works on one contiguous stretch of sites (their y neighbours included).
-constexpr int kWgBlocks = 512;  // blocks per term (a multiple of 8)
+constexpr int kWgBlocks = 1024;  // blocks per term (a multiple of 8)
 
 template <int MT, int NT>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
