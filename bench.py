@@ -51,19 +51,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(U_host, b_host, iters_gpu_plain, sample_iters, threads):
-    """FP64 OpenMP host oracle on a bounded sample: `sample_iters` BiCGSTAB
-    iterations of the same system, extrapolated to the iteration count of the
-    GPU's plain FP64 solve."""
+def host_info():
+    """the host cores the CPU baseline ran on"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def cpu_baseline(U_host, b_host, threads, checks):
+    """The FP64 OpenMP host oracle solves the same 32^3x64 clover system to the
+    same 1e-10 true residual (measured, not extrapolated), and recomputes the
+    true residual of each timed GPU solution in `checks` {name: x}."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
     if threads:
         oracle.set_threads(threads)
     nthreads = oracle.threads()
-    _, info = oracle.bicgstab(b_host, U_host, L, KAPPA, CSW, TB, tol=1e-30, max_iter=sample_iters)
-    per_iter = info["iter_s"] / max(info["iters"], 1)
-    est = info["setup_s"] + per_iter * iters_gpu_plain
+    t0 = time.perf_counter()
+    _, info = oracle.bicgstab(b_host, U_host, L, KAPPA, CSW, TB, tol=0.5 * TOL)
+    wall = time.perf_counter() - t0
     # the host Wilson hop alone (one parity), median of 3
     ts = []
     for _ in range(3):
@@ -72,13 +86,18 @@ def cpu_baseline(U_host, b_host, iters_gpu_plain, sample_iters, threads):
         ts.append(time.perf_counter() - t0)
     hop_s = sorted(ts)[1]
     vh = b_host.shape[1]
-    return {"value": round(est, 3), "unit": "s (FP64 time-to-solution, extrapolated)",
+    per_iter = info["iter_s"] / max(info["iters"], 1)
+    true_res = {name: oracle.true_residual(x, b_host, U_host, L, KAPPA, CSW, TB) for name, x in checks.items()}
+    return {"value": round(info["setup_s"] + info["iter_s"], 3), "unit": "s (FP64 time-to-solution, measured)",
             "cores": nthreads, "kind": "port",
-            "sample": f"{info['iters']} BiCGSTAB iterations of the same 32^3x64 clover system "
-                      f"({info['iter_s']:.1f} s) + clover set-up ({info['setup_s']:.1f} s), "
-                      f"extrapolated to {iters_gpu_plain} iterations",
+            "sample": f"one full FP64 even-odd BiCGSTAB solve of the same 32^3x64 clover system and source "
+                      f"to true residual {info['true_res']:.2e} ({info['iters']} iterations, clover set-up "
+                      f"{info['setup_s']:.1f} s, iterations {info['iter_s']:.1f} s; {wall:.1f} s wall with I/O)",
+            "iters": info["iters"], "true_res": info["true_res"], "s_per_iteration": round(per_iter, 4),
+            "omp_threads": nthreads, "host": host_info(),
             "hop_fp64": {"ms": round(hop_s * 1e3, 2), "GBps": round(HOP_BYTES[0] * vh / hop_s / 1e9, 2),
-                         "GFLOPs": round(1320 * vh / hop_s / 1e9, 2)}}
+                         "GFLOPs": round(1320 * vh / hop_s / 1e9, 2)},
+            "oracle_true_res_of_gpu_solutions": true_res}
 
 
 def main():
@@ -86,7 +105,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=6, help="oracle iterations in the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner", default="fp16", choices=["fp32", "fp16"])
@@ -120,24 +138,24 @@ def main():
         log(f"  prec {p}: {lat.tune(p)}")
 
     def timed(fn, n):
-        ts, res = [], None
+        ts, res, x = [], None, None
         for _ in range(n):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _, res = fn()
+            x, res = fn()
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
             assert res.converged and res.true_res <= TOL, f"solve did not reach {TOL}: {res}"
-        return sorted(ts)[len(ts) // 2], res
+        return sorted(ts)[len(ts) // 2], res, x
 
     # plain FP64 BiCGSTAB and the mixed-precision solve without the network
     plain = lambda: lat.solve(b, tol=TOL, prec=FP64)
     mixed = lambda: lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta)
     plain()
     mixed()
-    t_plain, plain_res = timed(plain, max(1, args.steps))
+    t_plain, plain_res, x_plain = timed(plain, max(1, args.steps))
     log(f"plain FP64: {t_plain * 1e3:.1f} ms {plain_res}")
-    t_mixed, mixed_res = timed(mixed, max(1, args.steps))
+    t_mixed, mixed_res, _ = timed(mixed, max(1, args.steps))
     log(f"mixed {args.inner}: {t_mixed * 1e3:.1f} ms {mixed_res}")
 
     # the learned preconditioner, trained on the GPU on Gaussian odd-site sources
@@ -165,7 +183,7 @@ def main():
     t0 = time.perf_counter()
     lres = []
     for _ in range(args.steps):
-        _, r = learned()
+        x_learned, r = learned()
         lres.append(r)
     torch.cuda.synchronize()
     t_learned = (time.perf_counter() - t0) / args.steps
@@ -189,9 +207,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle, bounded sample)")
+        log("cpu baseline: the oracle's FP64 solve, and the oracle's true residual of the GPU solutions")
         cpu = cpu_baseline(U.cpu().numpy().reshape(4, 2, vh, 3, 3), b.cpu().numpy().reshape(2, vh, 4, 3),
-                           plain_res.iters, args.cpu_iters, args.cpu_threads)
+                           args.cpu_threads,
+                           {"plain": x_plain.cpu().numpy().reshape(2, vh, 4, 3),
+                            "learned_mixed": x_learned.cpu().numpy().reshape(2, vh, 4, 3)})
+        for name, tr in cpu["oracle_true_res_of_gpu_solutions"].items():
+            assert tr <= TOL, f"the oracle's true residual of the {name} solution is {tr:.3e}"
 
     out = {
         "metric": METRIC,

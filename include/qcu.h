@@ -165,6 +165,13 @@ int qcuBatchTune(QcuBatchHandle h, int prec, double *best_ms);
  * precision and keeps the fastest; the table persists in a text file */
 int qcuTune(QcuLatticeHandle h, int prec, double *best_ms);
 int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group);
+/* pins the launch shape of the hop kernel for this lattice (or batch) and
+ * precision instead of tuning it: block 64, 128 or 256 threads; remap 0/1
+ * (XCD-aware block order off/on); group 0, 1 or 2 (scheduling fences after
+ * none / every / every second direction).  Solver graphs captured with an
+ * earlier shape are re-captured. */
+int qcuTuneSet(QcuLatticeHandle h, int prec, int block, int remap, int group);
+int qcuBatchTuneSet(QcuBatchHandle h, int prec, int block, int remap, int group);
 int qcuTuneSave(const char *path);
 int qcuTuneLoad(const char *path);
 
@@ -210,6 +217,13 @@ int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss);
 int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *flops);
 /* out = K in on one-parity (odd) external fields, as applied inside the solver */
 int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);
+/* weights file: one line "QCUNET1 {json metadata}" (channels, weight count,
+ * dtype, and the lattice, kappa, c_sw and boundary the network was trained
+ * on), then the weights as little-endian FP32 (re, im) pairs in the order of
+ * qcuNetGetWeights.  qcuNetLoad requires the same channel list and resets the
+ * optimiser state; the weights act site by site, so any lattice can load them. */
+int qcuNetSave(QcuNetHandle h, const char *path);
+int qcuNetLoad(QcuNetHandle h, const char *path);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,8 @@ SIGNATURES = {
     "qcuBatchTune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "qcuTuneGet": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_int)]),
+    "qcuTuneSet": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "qcuBatchTuneSet": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "qcuTuneSave": (C.c_int, [C.c_char_p]),
     "qcuTuneLoad": (C.c_int, [C.c_char_p]),
     # learned preconditioner
@@ -96,6 +98,8 @@ SIGNATURES = {
     "qcuNetApplyOdd": (C.c_int, [_VP, _VP, _VP]),
     "qcuNetLossGrad": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
     "qcuNetBenchLayer": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "qcuNetSave": (C.c_int, [_VP, C.c_char_p]),
+    "qcuNetLoad": (C.c_int, [_VP, C.c_char_p]),
 }
 
 

@@ -1,6 +1,12 @@
-"""Benchmarks of the smaller workloads (time-to-solution at 1e-10 true residual).
+"""Benchmarks of the smaller workloads (time-to-solution at 1e-10 true
+residual) and the kernel measurements behind the rocprofv3 summaries.
 
     python -m qcu_amd.configbench [c1] [c2] [c3a] [c3b] [c4] [legacy]
+    python -m qcu_amd.configbench hop [--prec P --clover 0/1 --xpay 0/1 --iters N]
+    python -m qcu_amd.configbench net-layer [--layer K --iters N]
+    python -m qcu_amd.configbench net-apply [--iters N]
+    python -m qcu_amd.configbench net-backward [--iters N]
+    python -m qcu_amd.configbench copy [--mib M --iters N]
 
 c1  plain clover BiCGSTAB FP64, 16^3x32, eps 0.3, csw 1.0, kappa 0.124, point
     source (origin, spin 0, colour 0): iterations, time, FP64 hop GB/s.
@@ -21,6 +27,18 @@ legacy  the reference's entry points through qcu_amd.pyqcu on the c1 lattice
     the links and, for clover, builds the clover term), against the resident
     FP64 hop of the handle API; ncclBistabCgQcu (kappa 0.125, periodic, stop
     on Re<r,r> < 1e-9) with its |b - M x|^2.
+
+Kernel measurements, all on the 32^3x64 clover lattice of the headline
+benchmark (eps 0.2, csw 1.0, kappa 0.1248) unless --L is given:
+hop          the resident hop kernel (qcuBenchHop): HIP-event ms per launch and
+             algorithmic GB/s.
+net-layer    one forward layer of the 1-8-8-8-1 network (qcuNetBenchLayer) on
+             the activations of a forward pass over Gaussian data.
+net-apply    the network applied as the preconditioner (odd field in, odd part
+             of the output kept; qcuNetApplyOdd), median wall ms.
+net-backward full forward + backward passes (qcuNetBackward), median wall ms.
+copy         a device-to-device copy and a fill of M MiB: the plain streams of
+             known size that scale the FETCH_SIZE / WRITE_SIZE counters.
 
 Every solve must reach |b - M x| / |b| <= 1e-10 (FP64); one JSON line per
 workload.  Times are medians of 3 (2 for batches) after a warm-up solve.
@@ -203,11 +221,115 @@ def legacy():
     return d
 
 
+def _headline_lattice(args):
+    L = tuple(args.L)
+    lat = Lattice(L, 0.1248, 1.0 if args.clover else 0.0, -1)
+    U = random_gauge(L, 0.2, 1)
+    lat.load_gauge(U)
+    del U
+    torch.cuda.empty_cache()
+    return lat
+
+
+def _median_ms(fn, n):
+    ts = []
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return sorted(ts)[len(ts) // 2]
+
+
+def hop(args):
+    lat = _headline_lattice(args)
+    ms = lat.bench_hop(prec=args.prec, clover=bool(args.clover), xpay=bool(args.xpay), iters=args.iters)
+    alg = (HOP[args.prec] + (CL[args.prec] if args.clover else 0) + (XP[args.prec] if args.xpay else 0)) * lat.vh
+    return {"L": lat.L, "prec": args.prec, "clover": args.clover, "xpay": args.xpay,
+            "shape": lat.launch_shape(args.prec), "ms_per_launch": ms, "alg_bytes": alg,
+            "alg_GBps": alg / ms / 1e6}
+
+
+def _network(lat, seed=3, init_scale=0.3):
+    from .lattice import LearnedPreconditioner
+    return LearnedPreconditioner(lat, (1, 8, 8, 8, 1), seed=seed, init_scale=init_scale)
+
+
+def net_layer(args):
+    from .lattice import fermion_shape, gaussian
+    lat = _headline_lattice(args)
+    net = _network(lat, init_scale=0.05)
+    # one forward pass on Gaussian data fills every layer's input with
+    # realistic activations (all-zero operands draw less power and run the
+    # matrix cores at a higher clock than real data)
+    net.forward(gaussian(fermion_shape(lat.L), 7))
+    ms, fl = net.bench_layer(args.layer, iters=args.iters)
+    return {"L": lat.L, "net_layer": args.layer, "ms_per_launch": ms, "alg_flops": fl, "TFLOPs": fl / ms / 1e9}
+
+
+def net_apply(args):
+    from .lattice import gaussian
+    lat = _headline_lattice(args)
+    net = _network(lat)
+    x = gaussian((lat.vh, 4, 3), 7)
+    for _ in range(3):
+        net.apply_odd(x)
+    return {"L": lat.L, "net_apply_ms_median": _median_ms(lambda: net.apply_odd(x), args.iters)}
+
+
+def net_backward(args):
+    from .lattice import fermion_shape, gaussian
+    lat = _headline_lattice(args)
+    net = _network(lat)
+    x = gaussian((1,) + fermion_shape(lat.L), 5)
+    gout = gaussian((1,) + fermion_shape(lat.L), 6)
+    net.backward(x, gout)
+    return {"L": lat.L, "net_backward_ms_median": _median_ms(lambda: net.backward(x, gout), args.iters)}
+
+
+def copy(args):
+    n = args.mib * (1 << 20)
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    b.fill_(1)
+    torch.cuda.synchronize()
+
+    def per_launch(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.iters):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.iters
+
+    ms = per_launch(lambda: b.copy_(a))
+    ms_fill = per_launch(lambda: b.fill_(1))
+    return {"copy_bytes_read": n, "copy_bytes_written": n, "ms": ms, "GBps_rw": 2 * n / ms / 1e6,
+            "fill_ms": ms_fill, "fill_GBps": n / ms_fill / 1e6}
+
+
 def main():
-    which = sys.argv[1:] or ["c1", "c2", "c3a", "c3b"]
+    import argparse
     fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b, "c4": c4, "legacy": legacy}
-    for w in which:
-        print(json.dumps({"workload": w, **fns[w]()}), flush=True)
+    kernels = {"hop": hop, "net-layer": net_layer, "net-apply": net_apply, "net-backward": net_backward,
+               "copy": copy}
+    ap = argparse.ArgumentParser(description="workload and kernel measurements")
+    ap.add_argument("which", nargs="*", default=["c1", "c2", "c3a", "c3b"],
+                    choices=sorted(fns) + sorted(kernels))
+    ap.add_argument("--L", type=int, nargs=4, default=[32, 32, 32, 64])
+    ap.add_argument("--prec", type=int, default=2)
+    ap.add_argument("--clover", type=int, default=1)
+    ap.add_argument("--xpay", type=int, default=1)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--layer", type=int, default=1)
+    ap.add_argument("--mib", type=int, default=1024)
+    args = ap.parse_args()
+    for w in args.which:
+        res = kernels[w](args) if w in kernels else fns[w]()
+        print(json.dumps({"workload": w, **res}), flush=True)
 
 
 if __name__ == "__main__":

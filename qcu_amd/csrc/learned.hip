@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/qcu.h"
@@ -1630,6 +1631,14 @@ int net_precond(void *handle, const Lattice *lat, Precond **out) {
   *out = &h->net;
   return 0;
 }
+
+// the network runs on its lattice's resident FP32 links: a lattice whose gauge
+// (re)load failed has none
+static int net_ready(const Net &net) {
+  QCU_CHECK(net.lat->gauge_loaded && ((net.lat->prec_mask >> P32) & 1), QCU_ERR_STATE,
+            "net: the lattice has no resident gauge field");
+  return 0;
+}
 }  // namespace qcu
 
 extern "C" {
@@ -1683,8 +1692,11 @@ int qcuNetGetGradient(QcuNetHandle h, double *host, long n) {
 // full-lattice forward: in [1][2][Vh][12], out [1][2][Vh][12] complex128 (device)
 int qcuNetForward(QcuNetHandle h, void *out, const void *in) {
   clear_error();
-  QCU_CHECK(h && out && in, QCU_ERR_ARG, "null argument");
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_TRY(check_dev_ptr(out, "out"));
+  QCU_TRY(check_dev_ptr(in, "in"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh;
   const long n0 = (long)net.chan[0] * 2 * vh;
@@ -1704,8 +1716,12 @@ int qcuNetForward(QcuNetHandle h, void *out, const void *in) {
 // weight gradient (read with qcuNetGetGradient) and grad_in (may be null)
 int qcuNetBackward(QcuNetHandle h, void *grad_in, const void *grad_out, const void *in) {
   clear_error();
-  QCU_CHECK(h && grad_out && in, QCU_ERR_ARG, "null argument");
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_TRY(check_dev_ptr(grad_out, "grad_out"));
+  QCU_TRY(check_dev_ptr(in, "in"));
+  if (grad_in) QCU_TRY(check_dev_ptr(grad_in, "grad_in"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh;
   const long n0 = (long)net.chan[0] * 2 * vh;
@@ -1742,7 +1758,9 @@ int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int 
                 double *seconds) {
   clear_error();
   QCU_CHECK(h && srcs && n > 0 && epochs > 0 && batch > 0, QCU_ERR_ARG, "bad training arguments");
+  for (int k = 0; k < n; ++k) QCU_TRY(check_dev_ptr(srcs[k], "srcs[k]"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   Lattice &lat = *net.lat;
   hipStream_t s = lat.stream;
   const int vh = lat.g.vh, bs = 256;
@@ -1758,7 +1776,6 @@ int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int 
       const int b1 = std::min(n, b0 + batch);
       QCU_HIP(hipMemsetAsync(gsum.p, 0, net.nw * 8, s));
       for (int k = b0; k < b1; ++k) {
-        QCU_CHECK(srcs[k], QCU_ERR_ARG, "null source");
         hipLaunchKernelGGL(k_load_odd, dim3(nb), dim3(bs), 0, s, tr.odd(0), (const double2 *)srcs[k], vh);
         QCU_HIP(hipGetLastError());
         double loss = 0;
@@ -1790,6 +1807,7 @@ int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
   QCU_TRY(check_dev_ptr(out, "out"));
   QCU_TRY(check_dev_ptr(in, "in"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
@@ -1812,6 +1830,7 @@ int qcuNetLossGrad(QcuNetHandle h, const void *src, double *loss) {
   QCU_CHECK(h && loss, QCU_ERR_ARG, "null argument");
   QCU_TRY(check_dev_ptr(src, "src"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh, bs = 256;
   NetTrainer tr{net, s};
@@ -1830,6 +1849,7 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   QCU_CHECK(h && ms && iters > 0, QCU_ERR_ARG, "bad arguments");
   Net &net = h->net;
   QCU_CHECK(layer >= 0 && layer < net.nl, QCU_ERR_ARG, "layer out of range");
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int ci = net.chan[layer], co = net.chan[layer + 1];
   auto launch = [&]() {
@@ -1865,11 +1885,93 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   return 0;
 }
 
+// Weights file: one text line of JSON metadata, then the complex weights as
+// little-endian FP32 (re, im) pairs in the order of qcuNetGetWeights.
+//   QCUNET1 {"format": "qcu-net", "version": 1, "layers": 4, "channels": [1, 8, 8, 8, 1],
+//            "weights": 25344, "dtype": "complex64-le", ...}
+// The lattice extents, kappa, c_sw and boundary are recorded for reference;
+// the weights act site by site, so they load onto any lattice.
+int qcuNetSave(QcuNetHandle h, const char *path) {
+  clear_error();
+  QCU_CHECK(h && path, QCU_ERR_ARG, "null argument");
+  Net &net = h->net;
+  std::vector<float> f(2 * net.nw);
+  QCU_HIP(hipMemcpy(f.data(), net.W.p, net.nw * 8, hipMemcpyDeviceToHost));
+  std::string ch;
+  for (int k = 0; k <= net.nl; ++k) ch += (k ? ", " : "") + std::to_string(net.chan[k]);
+  const Lattice &l = *net.lat;
+  char meta[1024];
+  snprintf(meta, sizeof(meta),
+           "QCUNET1 {\"format\": \"qcu-net\", \"version\": 1, \"layers\": %d, \"channels\": [%s], "
+           "\"weights\": %zu, \"dtype\": \"complex64-le\", "
+           "\"weight_order\": \"layer, term (2 mu forward, 2 mu + 1 backward, 8 site), output channel, "
+           "input channel, spin row, spin column\", \"lattice\": [%d, %d, %d, %d], \"kappa\": %.17g, "
+           "\"csw\": %.17g, \"t_boundary\": %d, \"adam_steps\": %d}\n",
+           net.nl, ch.c_str(), net.nw, l.L[0], l.L[1], l.L[2], l.L[3], l.kappa, l.csw, l.g.tbc, net.adam_t);
+  FILE *fp = fopen(path, "wb");
+  QCU_CHECK(fp, QCU_ERR_ARG, std::string("cannot write ") + path);
+  const size_t n = strlen(meta);
+  const bool ok = fwrite(meta, 1, n, fp) == n && fwrite(f.data(), 4, f.size(), fp) == f.size();
+  const bool closed = fclose(fp) == 0;
+  QCU_CHECK(ok && closed, QCU_ERR_ARG, std::string("write failed: ") + path);
+  return 0;
+}
+
+int qcuNetLoad(QcuNetHandle h, const char *path) {
+  clear_error();
+  QCU_CHECK(h && path, QCU_ERR_ARG, "null argument");
+  Net &net = h->net;
+  FILE *fp = fopen(path, "rb");
+  QCU_CHECK(fp, QCU_ERR_ARG, std::string("cannot read ") + path);
+  std::string meta;
+  int c;
+  while ((c = fgetc(fp)) != EOF && c != '\n' && meta.size() < 4096) meta.push_back((char)c);
+  auto fail = [&](const std::string &why) {
+    fclose(fp);
+    set_error(QCU_ERR_ARG, std::string(path) + ": " + why);
+    return QCU_ERR_ARG;
+  };
+  if (c != '\n' || meta.compare(0, 8, "QCUNET1 ") != 0) return fail("not a qcu network file");
+  // channels and weight count must match this network
+  const size_t pc = meta.find("\"channels\": [");
+  if (pc == std::string::npos) return fail("no channel list");
+  std::vector<int> ch;
+  const char *q = meta.c_str() + pc + 13;
+  while (*q && *q != ']') {
+    char *end = nullptr;
+    const long v = strtol(q, &end, 10);
+    if (end == q) return fail("bad channel list");
+    ch.push_back((int)v);
+    q = end;
+    while (*q == ',' || *q == ' ') ++q;
+  }
+  bool same = (int)ch.size() == net.nl + 1;
+  for (int k = 0; same && k <= net.nl; ++k) same = ch[k] == net.chan[k];
+  if (!same) return fail("channel list differs from this network's");
+  const size_t pw = meta.find("\"weights\": ");
+  if (pw == std::string::npos || strtoull(meta.c_str() + pw + 11, nullptr, 10) != net.nw)
+    return fail("weight count differs from this network's");
+  std::vector<float> f(2 * net.nw);
+  if (fread(f.data(), 4, f.size(), fp) != f.size()) return fail("file ends before the last weight");
+  if (fgetc(fp) != EOF) return fail("trailing data after the weights");
+  fclose(fp);
+  for (float x : f)
+    QCU_CHECK(std::isfinite(x), QCU_ERR_ARG, std::string(path) + ": non-finite weight");
+  QCU_HIP(hipMemcpy(net.W.p, f.data(), net.nw * 8, hipMemcpyHostToDevice));
+  // a loaded network starts a fresh optimiser state
+  QCU_HIP(hipMemset(net.m.p, 0, net.nw * 8));
+  QCU_HIP(hipMemset(net.v.p, 0, net.nw * 8));
+  net.adam_t = 0;
+  return 0;
+}
+
 // mean loss over sources without training
 int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss) {
   clear_error();
   QCU_CHECK(h && srcs && n > 0 && loss, QCU_ERR_ARG, "bad arguments");
+  for (int k = 0; k < n; ++k) QCU_TRY(check_dev_ptr(srcs[k], "srcs[k]"));
   Net &net = h->net;
+  QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh, bs = 256;
   NetTrainer tr{net, s};

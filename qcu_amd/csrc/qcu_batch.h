@@ -20,6 +20,9 @@
 namespace qcu {
 
 constexpr int kMaxRhs = 16;  // right-hand sides per segment
+// virtual sites per segment: the kernels address a field with a 32-bit
+// per-lane byte offset of 16 bytes per virtual site
+constexpr long long kMaxVirtualSites = 1LL << 28;
 
 struct SegDesc {
   Geom g;

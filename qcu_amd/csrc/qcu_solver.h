@@ -29,6 +29,7 @@ struct VecSet {
   Spinor r, rt, p, v, s, t, tmp, x, ph, sh;
   std::map<long, hipGraphExec_t> graphs;
   unsigned long long graph_net = 0;  // uid of the preconditioner the graphs were captured with
+  unsigned long long graph_tune = 0; // tune_generation() when the graphs were captured
   int create(int prec, int px, int n, bool precond);
   ~VecSet();
 };
@@ -58,6 +59,8 @@ struct SolveStats {
 
 LaunchCfg tuned_cfg(const BatchOp &op, int prec);
 std::string tune_key(const BatchOp &op, int prec);
+// changes whenever a tuned launch shape is set (captured graphs hold the old one)
+unsigned long long tune_generation();
 
 // 0 when p is a HIP device (or managed) pointer; else records the error
 int check_dev_ptr(const void *p, const char *what);

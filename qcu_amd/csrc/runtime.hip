@@ -70,7 +70,10 @@ int BatchOp::build(const std::vector<const Lattice *> &ls, const std::vector<int
   for (size_t k = 0; k < ls.size(); ++k) {
     QCU_CHECK(ls[k] && ls[k]->gauge_loaded, QCU_ERR_STATE, "batch: gauge field not loaded");
     QCU_CHECK(nrhs[k] >= 1 && nrhs[k] <= kMaxRhs, QCU_ERR_ARG, "batch: 1 to 16 right-hand sides per lattice");
-    QCU_CHECK((long long)ls[k]->g.vh * nrhs[k] < (1LL << 30), QCU_ERR_DIMS, "batch: segment too large");
+    // kernels address a segment's fields with 32-bit per-lane byte offsets of
+    // 16 bytes per virtual site
+    QCU_CHECK((long long)ls[k]->g.vh * nrhs[k] < kMaxVirtualSites, QCU_ERR_DIMS,
+              "batch: segment too large (Vh * nrhs must be below 2^28)");
     SegDesc d = ls[k]->seg_desc(nrhs[k]);
     d.sys0 = nsys;
     d.voff = nvsites;
@@ -110,7 +113,8 @@ int Lattice::init(const int L_[4], double kappa_, double csw_, int tbc) {
   QCU_CHECK(L[0] % 2 == 0, QCU_ERR_DIMS, "Lx must be even (checkerboarded x)");
   QCU_CHECK(tbc == 1 || tbc == -1, QCU_ERR_ARG, "t_boundary must be +1 or -1");
   long vh = (long)L[0] * L[1] * L[2] * L[3] / 2;
-  QCU_CHECK(vh < (1L << 30), QCU_ERR_DIMS, "lattice too large for 32-bit site indices");
+  // kernels address fields with 32-bit per-lane byte offsets, 16 bytes per site
+  QCU_CHECK(vh < kMaxVirtualSites, QCU_ERR_DIMS, "lattice too large: Vh must be below 2^28");
   g.lxh = L[0] / 2;
   g.ly = L[1];
   g.lz = L[2];
@@ -130,7 +134,11 @@ int Lattice::load_gauge(const void *ext, int mask) {
   QCU_CHECK(ext != nullptr, QCU_ERR_ARG, "gauge pointer is null");
   const size_t vh = g.vh;
   mask |= 1;  // FP64 links are always kept (clover build, reliable updates)
-  prec_mask = mask;
+  // until every step below succeeds the lattice holds no usable gauge field:
+  // a failed (re)load leaves it unloaded rather than half rebuilt
+  gauge_loaded = false;
+  prec_mask = 0;
+  solver.reset();  // its batch table and graphs point at the previous state
   QCU_TRY(U64.alloc(8 * vh * 6 * 16));
   QCU_TRY(gauge_ext_to_internal(ext, g, P64, U64.p, nullptr, stream));
   if (mask & 2) {
@@ -165,9 +173,9 @@ int Lattice::load_gauge(const void *ext, int mask) {
     QCU_CHECK(hbad == 0, QCU_ERR_STATE, "singular clover block");
   }
   QCU_HIP(hipStreamSynchronize(stream));
-  gauge_loaded = true;
-  solver.reset();  // its batch table points at the previous buffers
+  prec_mask = mask;
   QCU_TRY(self.build({this}, {1}, stream));
+  gauge_loaded = true;
   return 0;
 }
 

@@ -165,6 +165,13 @@ static int run_chunk(Solver &S, VecSet &V, Precond *K, int chunk, bool use_graph
     for (int k = 0; k < chunk; ++k) QCU_TRY(enqueue_iteration(S, V, K));
     return 0;
   }
+  if (V.graph_tune != tune_generation()) {
+    // the graphs hold the launch shapes of their capture: a newly tuned (or
+    // set) shape needs new graphs
+    for (auto &g : V.graphs) (void)hipGraphExecDestroy(g.second);
+    V.graphs.clear();
+    V.graph_tune = tune_generation();
+  }
   if (K && V.graph_net != K->uid()) {
     // graphs captured with another preconditioner hold its device buffers: drop them all
     for (auto g = V.graphs.begin(); g != V.graphs.end();) {

@@ -5,6 +5,7 @@
 // instruction-scheduling groupings, and keeps the fastest.  Results persist in
 // a plain text cache file ("key block remap group" per line) so that later
 // processes skip the search.
+#include <atomic>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -19,7 +20,26 @@ namespace qcu {
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, LaunchCfg> g_tuned;
+std::atomic<unsigned long long> g_tune_gen{1};
 }  // namespace
+
+// changes whenever a tuned launch shape is set, so that captured solver
+// graphs (which hold the launch shapes of their capture) are re-captured
+unsigned long long tune_generation() { return g_tune_gen.load(); }
+
+static void set_tuned(const std::string &key, const LaunchCfg &c) {
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = c;
+  }
+  ++g_tune_gen;
+}
+
+static int valid_cfg(int block, int remap, int group) {
+  QCU_CHECK((block == 64 || block == 128 || block == 256) && (remap == 0 || remap == 1) && group >= 0 && group <= 2,
+            QCU_ERR_ARG, "launch shape: block 64, 128 or 256; remap 0 or 1; group 0, 1 or 2");
+  return 0;
+}
 
 std::string tune_key(const BatchOp &op, int prec) {
   std::string k;
@@ -97,10 +117,7 @@ static int tune_op(const BatchOp &op, hipStream_t s, int prec, double *best_ms, 
           best = c;
         }
       }
-  {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[tune_key(op, prec)] = best;
-  }
+  set_tuned(tune_key(op, prec), best);
   if (best_ms) *best_ms = bms;
   if (out) *out = best;
   return 0;
@@ -119,12 +136,42 @@ hipStream_t batch_stream(const QcuBatch_s *b);
 extern "C" {
 
 int qcuTune(QcuLatticeHandle h, int prec, double *best_ms) {
+  clear_error();
   QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
   QCU_CHECK(h->lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
   return tune_op(h->lat.self, h->lat.stream, prec, best_ms, nullptr);
 }
 
+int qcuTuneSet(QcuLatticeHandle h, int prec, int block, int remap, int group) {
+  clear_error();
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_CHECK(h->lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
+  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
+  QCU_TRY(valid_cfg(block, remap, group));
+  LaunchCfg c;
+  c.block = block;
+  c.remap = remap;
+  c.group = group;
+  set_tuned(tune_key(h->lat.self, prec), c);
+  return 0;
+}
+
+int qcuBatchTuneSet(QcuBatchHandle h, int prec, int block, int remap, int group) {
+  clear_error();
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
+  QCU_TRY(valid_cfg(block, remap, group));
+  LaunchCfg c;
+  c.block = block;
+  c.remap = remap;
+  c.group = group;
+  set_tuned(tune_key(batch_op(h), prec), c);
+  return 0;
+}
+
 int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group) {
+  clear_error();
   QCU_CHECK(h && block && remap && group, QCU_ERR_ARG, "null argument");
   LaunchCfg c = tuned_cfg(h->lat.self, prec);
   *block = c.block;
@@ -134,7 +181,9 @@ int qcuTuneGet(QcuLatticeHandle h, int prec, int *block, int *remap, int *group)
 }
 
 int qcuBatchTune(QcuBatchHandle h, int prec, double *best_ms) {
+  clear_error();
   QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
   return tune_op(batch_op(h), batch_stream(h), prec, best_ms, nullptr);
 }
 
@@ -166,6 +215,7 @@ int qcuTuneLoad(const char *path) {
     }
   }
   fclose(f);
+  ++g_tune_gen;
   return 0;
 }
 

@@ -28,6 +28,24 @@ def _ptr(t: torch.Tensor) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
+def _check_field(t, n_complex: int, what: str):
+    """A field handed to the library: a contiguous complex128 tensor of exactly
+    n_complex elements (the library reads and writes that many)."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != torch.complex128:
+        raise ValueError(f"{what}: expected complex128, got {t.dtype}")
+    if t.numel() != n_complex:
+        raise ValueError(f"{what}: expected {n_complex} complex values, got {t.numel()}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous tensor")
+
+
+def _field_ptr(t, n_complex: int, what: str) -> C.c_void_p:
+    _check_field(t, n_complex, what)
+    return _ptr(t)
+
+
 def vol(L):
     return int(L[0] * L[1] * L[2] * L[3])
 
@@ -103,39 +121,54 @@ class Lattice:
     def vh(self):
         return vol(self.L) // 2
 
+    @property
+    def n_full(self):
+        """complex values of a full-lattice spinor field"""
+        return 12 * vol(self.L)
+
+    @property
+    def n_half(self):
+        """complex values of a one-parity spinor field"""
+        return 6 * vol(self.L)
+
     def load_gauge(self, U: torch.Tensor):
-        assert U.dtype == torch.complex128 and U.numel() == 36 * vol(self.L)
-        _lib.check(self._lib.qcuLatticeLoadGauge(self.h, _ptr(U)))
+        pu = _field_ptr(U, 36 * vol(self.L), "gauge")
+        _lib.check(self._lib.qcuLatticeLoadGauge(self.h, pu))
         torch.cuda.synchronize()
 
     def dslash(self, inp: torch.Tensor, parity: int, clover=False, prec=FP64) -> torch.Tensor:
+        pin = _field_ptr(inp, self.n_half, "dslash input")
         out = torch.empty_like(inp)
-        _lib.check(self._lib.qcuDslash(self.h, _ptr(out), _ptr(inp), int(parity), int(clover),
+        _lib.check(self._lib.qcuDslash(self.h, _ptr(out), pin, int(parity), int(clover),
                                        int(prec)))
         return out
 
     def mat(self, x: torch.Tensor) -> torch.Tensor:
+        px = _field_ptr(x, self.n_full, "mat input")
         out = torch.empty_like(x)
-        _lib.check(self._lib.qcuMatFull(self.h, _ptr(out), _ptr(x)))
+        _lib.check(self._lib.qcuMatFull(self.h, _ptr(out), px))
         return out
 
     def true_residual(self, x, b) -> float:
+        px, pb = _field_ptr(x, self.n_full, "x"), _field_ptr(b, self.n_full, "b")
         r = C.c_double()
-        _lib.check(self._lib.qcuTrueResidual(self.h, _ptr(x), _ptr(b), C.byref(r)))
+        _lib.check(self._lib.qcuTrueResidual(self.h, px, pb, C.byref(r)))
         return r.value
 
     def solve(self, b, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
               use_graph=True, precond=None, legacy_stop=False, x=None, method="bicgstab"):
         """x with |b - M x| / |b| <= tol; method "bicgstab" (any precision,
         optional preconditioner) or "cgnr" (CG on the normal equations, FP64)"""
+        pb = _field_ptr(b, self.n_full, "b")
         if x is None:
             x = torch.empty_like(b)
+        px = _field_ptr(x, self.n_full, "x")
         p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec),
                        reliable_delta=reliable_delta, chunk=chunk, use_graph=int(use_graph),
                        precond=None if precond is None else precond.handle,
                        legacy_stop=int(legacy_stop), method=METHODS[method])
         info = SolveInfo()
-        _lib.check(self._lib.qcuSolve(self.h, _ptr(x), _ptr(b), C.byref(p), C.byref(info)))
+        _lib.check(self._lib.qcuSolve(self.h, px, pb, C.byref(p), C.byref(info)))
         return x, SolveResult(info.iters, info.reliable_updates, bool(info.converged),
                               bool(info.fallback), info.true_res, info.seconds, info.restarts)
 
@@ -151,9 +184,17 @@ class Lattice:
     def tune(self, prec=FP64):
         ms = C.c_double()
         _lib.check(self._lib.qcuTune(self.h, int(prec), C.byref(ms)))
+        return dict(self.launch_shape(prec), ms=ms.value)
+
+    def launch_shape(self, prec=FP64):
+        """the hop kernel's launch shape used for this lattice and precision"""
         b, r, g = C.c_int(), C.c_int(), C.c_int()
         _lib.check(self._lib.qcuTuneGet(self.h, int(prec), C.byref(b), C.byref(r), C.byref(g)))
-        return {"block": b.value, "remap": r.value, "group": g.value, "ms": ms.value}
+        return {"block": b.value, "remap": r.value, "group": g.value}
+
+    def set_launch_shape(self, prec, block, remap, group):
+        """pin the hop kernel's launch shape (as the tuner would choose it)"""
+        _lib.check(self._lib.qcuTuneSet(self.h, int(prec), int(block), int(remap), int(group)))
 
 
 class Batch:
@@ -191,11 +232,13 @@ class Batch:
 
     def solve(self, bs, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
               use_graph=True, xs=None, method="bicgstab"):
-        assert len(bs) == self.nsys
+        if len(bs) != self.nsys or (xs is not None and len(xs) != self.nsys):
+            raise ValueError(f"expected {self.nsys} right-hand sides (and solutions)")
+        sizes = [lat.n_full for lat, n in zip(self.lattices, self.nrhs) for _ in range(n)]
+        bp = (C.c_void_p * self.nsys)(*[_field_ptr(b, n, "b").value for b, n in zip(bs, sizes)])
         if xs is None:
             xs = [torch.empty_like(b) for b in bs]
-        bp = (C.c_void_p * self.nsys)(*[_ptr(b).value for b in bs])
-        xp = (C.c_void_p * self.nsys)(*[_ptr(x).value for x in xs])
+        xp = (C.c_void_p * self.nsys)(*[_field_ptr(x, n, "x").value for x, n in zip(xs, sizes)])
         p = SolveParam(tol=tol, max_iter=max_iter, precision=int(prec), reliable_delta=reliable_delta,
                        chunk=chunk, use_graph=int(use_graph), precond=None, legacy_stop=0,
                        method=METHODS[method])
@@ -214,6 +257,10 @@ class Batch:
         ms = C.c_double()
         _lib.check(self._lib.qcuBatchTune(self.h, int(prec), C.byref(ms)))
         return ms.value
+
+    def set_launch_shape(self, prec, block, remap, group):
+        """pin the batched hop kernel's launch shape"""
+        _lib.check(self._lib.qcuBatchTuneSet(self.h, int(prec), int(block), int(remap), int(group)))
 
 
 def tune_save(path):
@@ -300,30 +347,42 @@ class LearnedPreconditioner:
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
         """K x on a full-lattice field"""
+        # the library requires one input and one output channel
+        px = _field_ptr(x, self.lattice.n_full, "network input")
         out = torch.empty_like(x)
-        _lib.check(self._lib.qcuNetForward(self.handle, _ptr(out), _ptr(x)))
+        _lib.check(self._lib.qcuNetForward(self.handle, _ptr(out), px))
         return out
 
     def backward(self, x: torch.Tensor, grad_out: torch.Tensor, want_input_grad=True):
         """(weight gradient as complex128 numpy, input gradient or None) for a
         real loss with dL/dRe(Kx) + i dL/dIm(Kx) = grad_out"""
+        px = _field_ptr(x, self.lattice.n_full, "network input")
+        pg = _field_ptr(grad_out, self.lattice.n_full, "output gradient")
         gi = torch.empty_like(x) if want_input_grad else None
-        _lib.check(self._lib.qcuNetBackward(self.handle, None if gi is None else _ptr(gi),
-                                            _ptr(grad_out), _ptr(x)))
+        _lib.check(self._lib.qcuNetBackward(self.handle, None if gi is None else _ptr(gi), pg, px))
         return self.gradient(), gi
 
     def apply_odd(self, x_odd: torch.Tensor) -> torch.Tensor:
         """K on an odd-site field (even sites taken as zero), as in the solver"""
+        px = _field_ptr(x_odd, self.lattice.n_half, "odd input")
         out = torch.empty_like(x_odd)
-        _lib.check(self._lib.qcuNetApplyOdd(self.handle, _ptr(out), _ptr(x_odd)))
+        _lib.check(self._lib.qcuNetApplyOdd(self.handle, _ptr(out), px))
         return out
+
+    def save(self, path):
+        """write the weights with their metadata (see qcuNetSave)"""
+        _lib.check(self._lib.qcuNetSave(self.handle, str(path).encode()))
+
+    def load(self, path):
+        """read weights written by save() (same channel list)"""
+        _lib.check(self._lib.qcuNetLoad(self.handle, str(path).encode()))
 
     def train(self, sources, epochs=10, batch=16, lr=1e-3):
         """Adam on |Mhat K b - b|^2 / |b|^2 over odd-site sources; returns
         (per-epoch mean losses, seconds)"""
         import numpy as np
         n = len(sources)
-        sp = (C.c_void_p * n)(*[_ptr(s).value for s in sources])
+        sp = (C.c_void_p * n)(*[_field_ptr(s, self.lattice.n_half, "source").value for s in sources])
         losses = np.zeros(epochs)
         secs = C.c_double()
         _lib.check(self._lib.qcuNetTrain(self.handle, sp, n, int(epochs), int(batch), float(lr),
@@ -342,6 +401,9 @@ class LearnedPreconditioner:
         import numpy as np
         if len(sources) != len(solutions) or not sources:
             raise ValueError("one solution per source")
+        for c, y in zip(sources, solutions):
+            _check_field(c, self.lattice.n_half, "source")
+            _check_field(y, self.lattice.n_half, "solution")
         w = float(solution_weight)
         n, vh = len(sources), self.lattice.vh
         full = (1,) + fermion_shape(self.lattice.L)
@@ -389,8 +451,9 @@ class LearnedPreconditioner:
 
     def loss_grad(self, source):
         """(loss, weight gradient) of |Mhat K b - b|^2 / |b|^2 for one odd source"""
+        ps = _field_ptr(source, self.lattice.n_half, "source")
         out = C.c_double()
-        _lib.check(self._lib.qcuNetLossGrad(self.handle, _ptr(source), C.byref(out)))
+        _lib.check(self._lib.qcuNetLossGrad(self.handle, ps, C.byref(out)))
         return out.value, self.gradient()
 
     def bench_layer(self, layer, iters=10):
@@ -401,7 +464,7 @@ class LearnedPreconditioner:
 
     def loss(self, sources) -> float:
         n = len(sources)
-        sp = (C.c_void_p * n)(*[_ptr(s).value for s in sources])
+        sp = (C.c_void_p * n)(*[_field_ptr(s, self.lattice.n_half, "source").value for s in sources])
         out = C.c_double()
         _lib.check(self._lib.qcuNetLoss(self.handle, sp, n, C.byref(out)))
         return out.value

@@ -106,3 +106,28 @@ def test_argument_errors_are_reported_without_a_device(qlib):
     assert lib.qcuGetLastError() == ERR_ARG and lib.qcuGetLastErrorString()
     chan = (C.c_int * 3)(1, 8, 1)
     assert lib.qcuNetCreate(C.byref(C.c_void_p()), null, 2, chan, 1, 0.05) == ERR_ARG
+
+
+def test_lattice_size_limit_of_32_bit_offsets(qlib):
+    """kernels address fields with 32-bit per-lane byte offsets of 16 bytes per
+    site, so a lattice needs Vh < 2^28: 64x64x256x512 (Vh = 2^28) is refused
+    with ERR_DIMS before any device call; one just below passes the check"""
+    ERR_DIMS = 2
+    lib = qlib.load()
+    h = C.c_void_p()
+    assert lib.qcuLatticeCreate(C.byref(h), (C.c_int * 4)(64, 64, 256, 512), 0.12, 0.0, -1) == ERR_DIMS
+    assert "2^28" in lib.qcuGetLastErrorString().decode()
+    rc = lib.qcuLatticeCreate(C.byref(h), (C.c_int * 4)(64, 64, 256, 510), 0.12, 0.0, -1)
+    assert rc != ERR_DIMS  # without a GPU: ERR_STATE (no device); with one: created
+    if rc == 0:
+        lib.qcuLatticeDestroy(h)
+
+
+def test_new_entry_points_validate_without_a_device(qlib):
+    lib = qlib.load()
+    null = C.c_void_p()
+    ERR_ARG = 1
+    assert lib.qcuTuneSet(null, 0, 64, 1, 0) == ERR_ARG
+    assert lib.qcuBatchTuneSet(null, 0, 64, 1, 0) == ERR_ARG
+    assert lib.qcuNetSave(null, b"/nonexistent") == ERR_ARG
+    assert lib.qcuNetLoad(null, b"/nonexistent") == ERR_ARG

@@ -236,3 +236,49 @@ def test_schur_solutions_and_pair_training(orc):
     assert np.isfinite(losses).all() and after[0] < before[0]
     net.close()
     lat.close()
+
+
+def test_weights_file_round_trip(orc, tmp_path):
+    """qcuNetSave / qcuNetLoad: the weights come back bit for bit (FP32), the
+    loaded network applies the same K, the metadata line records the channel
+    list and lattice, and a file for other channels or a truncated file is
+    refused with ERR_ARG"""
+    import json
+    from qcu_amd._lib import QcuError
+    from qcu_amd.lattice import Lattice, LearnedPreconditioner
+    L, chan = (8, 4, 4, 8), (1, 8, 8, 8, 1)
+    U, lat, net, W = _setup(orc, L, chan, -1, 140)
+    path = tmp_path / "k.qcunet"
+    net.save(path)
+    raw = path.read_bytes()
+    head, _, body = raw.partition(b"\n")
+    assert head.startswith(b"QCUNET1 ")
+    meta = json.loads(head[8:])
+    assert meta["channels"] == list(chan) and meta["weights"] == net.nweights
+    assert meta["lattice"] == list(L) and meta["dtype"] == "complex64-le"
+    assert len(body) == 8 * net.nweights
+    assert np.array_equal(np.frombuffer(body, "<c8").astype(np.complex128), W)
+    # a second lattice of another shape loads the same weights
+    L2 = (4, 4, 4, 8)
+    U2 = orc.gauge_random(L2, 0.3, 141)
+    lat2 = Lattice(L2, 0.124, 0.0, -1)
+    lat2.load_gauge(dev(U2))
+    other = LearnedPreconditioner(lat2, chan, seed=142)
+    other.load(path)
+    assert np.array_equal(other.get_weights(), W)
+    x = orc.gauss((1, 2, orc.vol(L2) // 2, 4, 3), 143)
+    ref, _ = orc.net_forward(x, W, chan, U2, L2, -1)
+    e = rel(host(other.forward(dev(x))).ravel(), ref)
+    log(f"weights file: {len(raw)} bytes, reloaded network vs oracle {e:.1e}")
+    assert e < 2e-5
+    wrong = LearnedPreconditioner(lat2, (1, 4, 1), seed=144)
+    with pytest.raises(QcuError) as err:
+        wrong.load(path)
+    assert err.value.code == 1 and "channel" in str(err.value)
+    (tmp_path / "short.qcunet").write_bytes(raw[:-8])
+    with pytest.raises(QcuError) as err:
+        other.load(tmp_path / "short.qcunet")
+    assert err.value.code == 1
+    assert np.array_equal(other.get_weights(), W)  # a refused file leaves the weights unchanged
+    for n in (net, other, wrong):
+        n.close()
