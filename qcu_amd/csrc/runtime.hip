@@ -173,9 +173,14 @@ int Lattice::load_gauge(const void *ext, int mask) {
     QCU_CHECK(hbad == 0, QCU_ERR_STATE, "singular clover block");
   }
   QCU_HIP(hipStreamSynchronize(stream));
+  // the lattice's own one-system batch table is built from a loaded lattice
   prec_mask = mask;
-  QCU_TRY(self.build({this}, {1}, stream));
   gauge_loaded = true;
+  if (const int rc = self.build({this}, {1}, stream)) {
+    gauge_loaded = false;
+    prec_mask = 0;
+    return rc;
+  }
   return 0;
 }
 
