@@ -43,10 +43,10 @@
 #include "qcu_lattice.h"
 #include "qcu_reduce.h"
 #include "qcu_solver.h"
+#include "qcu_net.h"
 
 namespace qcu {
 
-typedef cplx<float> cf;
 
 constexpr int kNetMaxLayers = 8;
 constexpr int kNetMaxChan = 8;
@@ -82,107 +82,8 @@ struct NetLayerArgs {
   int ngroups;    // output-channel groups
   const SysState *st;
   WaveOrder ord;  // matrix-core kernel only
+  unsigned *out_max;  // matrix-core kernel: receives the largest |component| written (or null)
 };
-
-QCU_D SpinorPtr chan_field(const SpinorPtr &base, int c, int p, int vh) {
-  SpinorPtr f;
-  f.p = (char *)base.p + ((size_t)c * 2 + p) * (size_t)vh * 96;
-  f.norm = nullptr;
-  f.stride = vh;
-  return f;
-}
-
-// neighbour site (half index) and link for term t at output site i of parity p
-struct TermGeo {
-  int j;      // input site (half index)
-  int jp;     // input parity
-  int hop;    // 0: site term
-  int fwd;
-  int slab;   // link slab (parity * 4 + mu)
-  int li;     // link site
-  float ph;
-};
-
-QCU_D TermGeo term_geo(const Geom &g, const Site &s, int i, int p, int t) {
-  TermGeo r;
-  r.j = i;
-  r.jp = p;
-  r.hop = t < 8;
-  r.fwd = !(t & 1);
-  r.slab = 0;
-  r.li = i;
-  r.ph = 1.f;
-  if (!r.hop) return r;
-  const int mu = t >> 1;
-  const int q = 1 - p;
-  const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
-  int j = i;
-  if (mu == 0) {
-    const int xe = 1 - s.xodd;
-    j = r.fwd ? i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh : i - xe + xe * (int)(s.xh == 0) * g.lxh;
-  } else if (mu == 1) {
-    j = r.fwd ? i + sy - (int)(s.y == g.ly - 1) * g.ly * sy : i - sy + (int)(s.y == 0) * g.ly * sy;
-  } else if (mu == 2) {
-    j = r.fwd ? i + sz - (int)(s.z == g.lz - 1) * g.lz * sz : i - sz + (int)(s.z == 0) * g.lz * sz;
-  } else {
-    j = r.fwd ? i + st_ - (int)(s.t == g.lt - 1) * g.lt * st_ : i - st_ + (int)(s.t == 0) * g.lt * st_;
-    const bool cross = r.fwd ? (s.t == g.lt - 1) : (s.t == 0);
-    r.ph = (cross && g.tbc < 0) ? -1.f : 1.f;
-  }
-  r.j = j;
-  r.jp = q;
-  r.slab = r.fwd ? p * 4 + mu : q * 4 + mu;
-  r.li = r.fwd ? i : j;
-  return r;
-}
-
-// the colour matrix of a term, applied as v <- M v: M = ph U for a forward
-// hop and ph U^dag for a backward one, from the two stored rows of U; the
-// identity for the site term
-QCU_D void term_matrix(const float4 (&raw)[3], int hop, int fwd, float ph, cf (&m)[9]) {
-  if (!hop) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) m[k] = mk(k % 4 == 0 ? 1.f : 0.f, 0.f);
-    return;
-  }
-  cf u[9];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    u[2 * k] = mk(raw[k].x, raw[k].y);
-    u[2 * k + 1] = mk(raw[k].z, raw[k].w);
-  }
-  recon_row2(u);
-  if (fwd) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) m[k] = mk(ph * u[k].re, ph * u[k].im);
-  } else {
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) m[r * 3 + c] = mk(ph * u[c * 3 + r].re, -ph * u[c * 3 + r].im);
-  }
-}
-
-// v <- M v as a chain of fused multiply-adds
-QCU_D void transport1(const cf (&m)[9], cf (&v)[3]) {
-  cf a[3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    float re = m[r * 3].re * v[0].re, im = m[r * 3].re * v[0].im;
-    re = fmaf(-m[r * 3].im, v[0].im, re);
-    im = fmaf(m[r * 3].im, v[0].re, im);
-#pragma unroll
-    for (int c = 1; c < 3; ++c) {
-      re = fmaf(m[r * 3 + c].re, v[c].re, re);
-      re = fmaf(-m[r * 3 + c].im, v[c].im, re);
-      im = fmaf(m[r * 3 + c].re, v[c].im, im);
-      im = fmaf(m[r * 3 + c].im, v[c].re, im);
-    }
-    a[r] = mk(re, im);
-  }
-#pragma unroll
-  for (int r = 0; r < 3; ++r) v[r] = a[r];
-}
 
 // the colour matrix of term tg at this site (FP32 links)
 QCU_D void term_link_matrix(const LinkPtr &U, const TermGeo &tg, cf (&m)[9]) {
@@ -510,21 +411,26 @@ QCU_D void net_layer_mfma_body(const NetLayerArgs &a, const float *Apk) {
     tg = ng;
     cur = nxt;
   }
-  if (!ok) return;
+  float omax = 0.f;
+  if (ok) {
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int o = 4 * mt + gq;
-    if (o >= a.cout) continue;
-    cf out[12];
+    for (int mt = 0; mt < MT; ++mt) {
+      const int o = 4 * mt + gq;
+      if (o >= a.cout) continue;
+      cf out[12];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float a = p1[c][mt][j], b = p2[c][mt][j];
-        out[j * 3 + c] = mk(a - b, p3[c][mt][j] - a - b);
-      }
-    SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
+        for (int c = 0; c < 3; ++c) {
+          const float a = p1[c][mt][j], b = p2[c][mt][j];
+          out[j * 3 + c] = mk(a - b, p3[c][mt][j] - a - b);
+          omax = fmaxf(omax, fmaxf(fabsf(out[j * 3 + c].re), fabsf(out[j * 3 + c].im)));
+        }
+      SpinorIO<P32>::store(chan_field(a.out, o, p, g.vh), i, out);
+    }
   }
+  // the next layer's split-FP16 operands are scaled by this field's largest component
+  if (a.out_max) record_max(a.out_max, omax);
 }
 
 // CI: input channel count known at compile time (0: a.cin at run time)
@@ -1184,6 +1090,14 @@ __global__ void k_adam(float *w, float *m, float *v, const float *g, int n, floa
 }
 
 // ---------------------------------------------------------------- host side
+// operands of a layer on the split-FP16 kernel
+struct SplitIO {
+  const void *A;            // packed weights
+  const float *winv;        // inverse weight scale
+  const unsigned *in_max;   // largest |component| of the input field
+  unsigned *out_max;        // receives that of the output field
+};
+
 struct Net : Precond {
   Lattice *lat = nullptr;
   int nl = 0;
@@ -1203,7 +1117,33 @@ struct Net : Precond {
   DevBuf mixb;      // term fields of the two-pass single-output layer (9 fields)
   int block = 128;  // threads per block of the layer kernel
   int tile = 8;     // rows per tile side of the matrix-core wave order (0: site order)
+  // split-FP16 hidden layers (8 -> 4 or 8 channels; QCU_NET_SPLIT=0 keeps them
+  // on the FP32 matrix-core kernel)
+  bool use_split = true;
+  int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
+  DevBuf spk;                   // packed operands: [direction][layer] x kSplitPackBytes
+  DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
+  DevBuf swinv;                 // [direction][layer] inverse weight scale
+  unsigned long long wver = 1;  // weight version (bumped on every change of W)
+  unsigned long long packed_ver = 0;
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
+  // layer k (or its transpose, dir 1: cin <-> cout) runs on the split kernel
+  bool split_layer(int k, int dir) const {
+    const int ci = dir ? chan[k + 1] : chan[k], co = dir ? chan[k] : chan[k + 1];
+    return use_mfma && use_split && ci == 8 && (co == 4 || co == 8);
+  }
+  SplitIO split_io(int k, int dir) {
+    SplitIO io;
+    io.A = (const char *)spk.p + ((size_t)dir * kNetMaxLayers + k) * kSplitPackBytes;
+    io.winv = (const float *)swinv.p + dir * kNetMaxLayers + k;
+    // forward: input act[k], output act[k + 1]; transpose: input g[k + 1], output g[k]
+    unsigned *mx = (unsigned *)smax.p + dir * (kNetMaxLayers + 1);
+    io.in_max = dir ? mx + k + 1 : mx + k;
+    io.out_max = dir ? mx + k : mx + k + 1;
+    return io;
+  }
+  int pack_split(int dir, hipStream_t s);
+  int reset_max(int dir, hipStream_t s);
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
   SpinorPtr act_ptr(int layer) const {
@@ -1213,8 +1153,10 @@ struct Net : Precond {
     s.stride = lat->g.vh;
     return s;
   }
+  // io: the split kernel's operands and field maxima (null: FP32 kernels only,
+  // no maximum recorded)
   int layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
-            const SysState *st, hipStream_t s, int zero_par = -1);
+            const SysState *st, hipStream_t s, int zero_par = -1, const SplitIO *io = nullptr);
   // zero_par0: parity on which the network input is zero (-1: none)
   int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
   // gout_zero_par: parity on which gout_last is zero; act0_zero_par: parity on
@@ -1223,7 +1165,9 @@ struct Net : Precond {
                int act0_zero_par = -1);
   int apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) override;
   unsigned long long id = 0;
-  unsigned long long uid() const override { return id; }
+  // changes with the weights too: solver graphs captured with other weights
+  // (or their packed operands) are re-captured
+  unsigned long long uid() const override { return (id << 32) ^ wver; }
 };
 
 static std::atomic<unsigned long long> g_net_ids{0};
@@ -1295,10 +1239,17 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
     for (int s = 0; s < 4; ++s) h[2 * (woff[k] + (((size_t)8 * co + 0) * ci + 0) * 16 + s * 5)] += 1.f;
   }
   QCU_HIP(hipMemcpy(W.p, h.data(), nw * 8, hipMemcpyHostToDevice));
+  ++wver;
   QCU_HIP(hipMemset(m.p, 0, nw * 8));
   QCU_HIP(hipMemset(v.p, 0, nw * 8));
   adam_t = 0;
   QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 2 * kApkParts * 64 * 4));
+  QCU_TRY(spk.alloc((size_t)2 * kNetMaxLayers * kSplitPackBytes));
+  QCU_TRY(smax.alloc((size_t)2 * (kNetMaxLayers + 1) * 4));
+  QCU_TRY(swinv.alloc((size_t)2 * kNetMaxLayers * 4));
+  QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
+  if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
+  if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
   // a layer with one output and several input channels (the last layer, and
@@ -1319,7 +1270,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
 }
 
 int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
-               const SysState *st, hipStream_t s, int zero_par) {
+               const SysState *st, hipStream_t s, int zero_par, const SplitIO *io) {
   NetLayerArgs a;
   a.in = in;
   a.out = out;
@@ -1331,6 +1282,24 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
   a.parity0 = parity0;
   a.zero_par = zero_par;
   a.st = st;
+  a.out_max = io ? io->out_max : nullptr;
+  // a hidden layer with packed split operands: FP16 matrix cores
+  if (io && io->A && use_mfma && use_split && cin == 8 && (cout == 4 || cout == 8) && zero_par < 0) {
+    NetSplitArgs b;
+    b.in = (const char *)in.p;
+    b.out = (char *)out.p;
+    b.A = (const uint4 *)io->A;
+    b.winv = io->winv;
+    b.in_max = io->in_max;
+    b.out_max = io->out_max;
+    b.U = a.U;
+    b.g = lat->g;
+    b.parity0 = parity0;
+    b.npar = nparity;
+    b.order = split_order;
+    b.st = st;
+    return net_layer_split(b, cout, s);
+  }
   // one output channel: the input channels are mixed per term on the matrix
   // cores at the input sites, then transported once per term
   if (use_mfma && cout == 1 && cin > 1 && zero_par < 0) {
@@ -1424,12 +1393,49 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
 }
 
 // act[0] holds the input; fills act[1..nl] (act[nl] = output)
+// Packs the split operands of every split layer in direction dir (0: the
+// weights, 1: their transposes in Wt) and records the weight version packed.
+int Net::pack_split(int dir, hipStream_t s) {
+  for (int k = 0; k < nl; ++k) {
+    if (!split_layer(k, dir)) continue;
+    const SplitIO io = split_io(k, dir);
+    const float2 *w = (const float2 *)(dir ? Wt.p : W.p) + woff[k];
+    QCU_TRY(net_pack_split(w, dir ? chan[k + 1] : chan[k], dir ? chan[k] : chan[k + 1], (void *)io.A,
+                           (float *)io.winv, s));
+  }
+  if (dir == 0) packed_ver = wver;
+  return 0;
+}
+
+__global__ void k_fill_u32(unsigned *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// clears the field maxima of direction dir before a pass writes them
+int Net::reset_max(int dir, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (unsigned *)smax.p + dir * (kNetMaxLayers + 1),
+                     kNetMaxLayers + 1);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
 int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0) {
+  bool any = false;
+  for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
+  if (any) {
+    if (packed_ver != wver) QCU_TRY(pack_split(0, s));
+    QCU_TRY(reset_max(0, s));
+  }
   for (int k = 0; k < nl; ++k) {
     const bool last = k == nl - 1;
     const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
+    // the kernel before a split layer records the largest component it writes
+    SplitIO io = split_io(k, 0);
+    if (!split_layer(k, 0)) io.A = nullptr;
+    if (!(k + 1 < nl && split_layer(k + 1, 0))) io.out_max = nullptr;
     QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s,
-                  k == 0 ? zero_par0 : -1));
+                  k == 0 ? zero_par0 : -1, any ? &io : nullptr));
   }
   return 0;
 }
@@ -1444,6 +1450,12 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
                        (const float2 *)W.p + woff[k], (float2 *)Wt.p + woff[k], chan[k], chan[k + 1]);
   }
   QCU_HIP(hipGetLastError());
+  bool any = false;
+  for (int k = 0; k < nl; ++k) any = any || split_layer(k, 1);
+  if (any) {
+    QCU_TRY(pack_split(1, s));
+    QCU_TRY(reset_max(1, s));
+  }
   SpinorPtr gcur = gout;
   int ping = 0;
   for (int k = nl - 1; k >= 0; --k) {
@@ -1497,8 +1509,11 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
       gnext.norm = nullptr;
       gnext.stride = lat->g.vh;
     }
+    SplitIO io = split_io(k, 1);
+    if (!split_layer(k, 1)) io.A = nullptr;
+    if (!(k > 0 && split_layer(k - 1, 1))) io.out_max = nullptr;
     QCU_TRY(layer(gcur, gnext, (const float2 *)Wt.p + woff[k], cout, cin, 0, 2, nullptr, s,
-                  k == nl - 1 ? gout_zero_par : -1));
+                  k == nl - 1 ? gout_zero_par : -1, any ? &io : nullptr));
     gcur = gnext;
     ping ^= 1;
   }
@@ -1628,7 +1643,13 @@ namespace qcu {
 int net_precond(void *handle, const Lattice *lat, Precond **out) {
   QcuNet_s *h = (QcuNet_s *)handle;
   QCU_CHECK(h->net.lat == lat, QCU_ERR_ARG, "the preconditioner was built for another lattice");
-  *out = &h->net;
+  // pack the split operands now, on the lattice stream, rather than inside a
+  // captured solver iteration
+  Net &net = h->net;
+  bool any = false;
+  for (int k = 0; k < net.nl; ++k) any = any || net.split_layer(k, 0);
+  if (any && net.packed_ver != net.wver) QCU_TRY(net.pack_split(0, net.lat->stream));
+  *out = &net;
   return 0;
 }
 
@@ -1677,6 +1698,7 @@ int qcuNetSetWeights(QcuNetHandle h, const double *host, long n) {
   std::vector<float> f(2 * n);
   for (long k = 0; k < 2 * n; ++k) f[k] = (float)host[k];
   QCU_HIP(hipMemcpy(h->net.W.p, f.data(), n * 8, hipMemcpyHostToDevice));
+  ++h->net.wver;
   return 0;
 }
 
@@ -1793,6 +1815,7 @@ int qcuNetTrain(QcuNetHandle h, const void *const *srcs, int n, int epochs, int 
                          (float *)net.v.p, (const float *)gsum.p, nfl, (float)lr, b1f, b2f, c1, c2, 1e-8f,
                          1.f / (float)(b1 - b0));
       QCU_HIP(hipGetLastError());
+      ++net.wver;
     }
     if (losses) losses[e] = esum / n;
   }
@@ -1852,9 +1875,15 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int ci = net.chan[layer], co = net.chan[layer + 1];
+  // a split layer runs on the operands packed from the current weights and
+  // the input maximum of the last forward pass
+  const bool split = net.split_layer(layer, 0);
+  if (split && net.packed_ver != net.wver) QCU_TRY(net.pack_split(0, s));
+  SplitIO io = net.split_io(layer, 0);
+  io.out_max = nullptr;
   auto launch = [&]() {
     return net.layer(net.act_ptr(layer), net.act_ptr(layer + 1), (const float2 *)net.W.p + net.woff[layer], ci, co, 0,
-                     2, nullptr, s);
+                     2, nullptr, s, -1, split ? &io : nullptr);
   };
   QCU_TRY(launch());
   hipEvent_t e0, e1;
@@ -1874,10 +1903,12 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
     // (16 complex multiply-adds each) + 8 hops x cin transports of 4 spins
     // (9 complex multiply-adds each).  A complex multiply-add is 8 real flops,
     // except in the matrix-core mixing, which uses three real products per
-    // complex product (6 flops).  The two-pass single-output layer transports
-    // one mixed field per hop term.
+    // complex product (6 flops); the split-FP16 kernel is counted at 8 flops,
+    // the FP32 arithmetic it reproduces (it executes three FP16 products of
+    // 8 flops each).  The two-pass single-output layer transports one mixed
+    // field per hop term.
     const bool two_pass = net.use_mfma && co == 1 && ci > 1;
-    const bool mfma = net.use_mfma && (co > 1 || two_pass);
+    const bool mfma = net.use_mfma && (co > 1 || two_pass) && !split;
     const double mix = (mfma ? 6.0 : 8.0) * 9.0 * co * ci * 16 * 3;
     const double per_site = mix + 8.0 * (8.0 * (two_pass ? 1 : ci) * 4 * 9);
     *flops = per_site * 2.0 * net.lat->g.vh;
@@ -1958,6 +1989,7 @@ int qcuNetLoad(QcuNetHandle h, const char *path) {
   for (float x : f)
     QCU_CHECK(std::isfinite(x), QCU_ERR_ARG, std::string(path) + ": non-finite weight");
   QCU_HIP(hipMemcpy(net.W.p, f.data(), net.nw * 8, hipMemcpyHostToDevice));
+  ++net.wver;
   // a loaded network starts a fresh optimiser state
   QCU_HIP(hipMemset(net.m.p, 0, net.nw * 8));
   QCU_HIP(hipMemset(net.v.p, 0, net.nw * 8));

@@ -1,0 +1,404 @@
+// net_split.hip -- the hidden layers of the learned preconditioner on the
+// FP16 matrix cores at FP32 accuracy (new; no counterpart in the reference).
+//
+// A hidden layer with 8 input and 4 or 8 output channels is, per colour c,
+// the complex GEMM
+//   D[m][n] = sum_k W[m][k] V[k][n],  m = (output channel o, spin s),
+//   k = (term t, input channel i, spin s2),  n = output site,
+// where V holds the input channels transported from the term's neighbour
+// (V_t(x) = M_t(x) in(neighbour_t(x)), M_t = +-U or +-U^dag, the identity for
+// the site term).  It runs as a real GEMM on v_mfma_f32_32x32x16_f16 with each
+// complex product embedded as a 2x2 real block,
+//   [Re D]   [Wr  -Wi] [Re V]
+//   [Im D] = [Wi   Wr] [Im V],
+// and every FP32 operand split into two FP16 numbers, x = hi + lo with hi the
+// FP16 rounding of x and lo that of the remainder x - hi, so that
+//   W V = Whi Vhi + Whi Vlo + Wlo Vhi + O(2^-22 |W| |V|)
+// in three FP16 products accumulated in FP32 (the split-operand method of
+// Ootomo & Yokota, "Recovering single precision accuracy from Tensor Cores
+// while surpassing the FP32 theoretical peak performance", IJHPCA 36 (2022);
+// the lo x lo product is dropped).  FP16 holds 11 significant bits over 30
+// binades, so both operands are scaled by exact powers of two first: the
+// weights by one factor per layer (largest entry below 2^14, set when they
+// are packed), the input field by one factor per application from its largest
+// component, which the kernel that wrote the field records; the factors are
+// divided out of D.  Values far below the largest lose their lo part to FP16
+// underflow: their absolute error stays below 2^-25 of the scaled unit, 2^-39
+// of the largest component.
+//
+// Lane layout of v_mfma_f32_32x32x16_f16 (64 lanes, g = lane / 32):
+//   A: lane holds A[m = lane % 32][k = 8 g + q], q = 0..7;
+//   B: lane holds B[k = 8 g + q][n = lane % 32];
+//   D: 16 floats, D[m = 8 (r / 4) + 4 g + r % 4][n = lane % 32], r = 0..15.
+// Index order chosen to match it:
+//   M tile mt (32 rows): m = 8 ol + 2 s + ri, output channel o = 4 mt + ol,
+//     spin s, ri = 0 (Re) / 1 (Im);
+//   K step ks (16): k = 8 il + 2 s2 + ri, input channel i = 2 ks + il.
+// So in K step ks lane group g transports input channel 2 ks + g of its site
+// (all four spins, one colour per product), and after the last step lane
+// (n, g) holds, for each output channel, spins 2g and 2g + 1 of site n: it
+// stores float4 planes 3g .. 3g + 2 of the site-fastest FP32 layout.
+//
+// Work: a wave owns 32 consecutive half-sites of one output parity (a chunk)
+// and runs the 9 terms x 4 K steps over them; the packed A operands of the
+// layer (9 x MT x 4 x {hi, lo} x 1 KiB = 144 KiB for MT = 2) stay in LDS, so a
+// block of 8 waves (two per SIMD) per CU loads them once and then walks a
+// sequence of chunks.  The channel operand of the next step is loaded while
+// the products of the current one run.
+#include <cmath>
+#include <cstdlib>
+
+#include "qcu_internal.h"
+#include "qcu_net.h"
+
+namespace qcu {
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// The two MFMA shapes.  NS sites per wave (the N of the product); the 64
+// lanes form G = 64 / NS groups, lane group g supplying K rows 8 g .. 8 g + 7.
+// NR accumulator registers per lane; row(r, g) the D row a lane's register r
+// holds.
+template <int NS>
+struct SplitShape;
+template <>
+struct SplitShape<32> {  // v_mfma_f32_32x32x16_f16
+  typedef f16v acc_t;
+  static constexpr int NR = 16;
+  static QCU_D acc_t mfma(h8v a, h8v b, acc_t c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+  static constexpr QCU_HD int row(int r, int g) { return 8 * (r / 4) + 4 * g + r % 4; }
+};
+template <>
+struct SplitShape<16> {  // v_mfma_f32_16x16x32_f16
+  typedef f4v acc_t;
+  static constexpr int NR = 4;
+  static QCU_D acc_t mfma(h8v a, h8v b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+  static constexpr QCU_HD int row(int r, int g) { return 4 * g + r; }
+};
+
+// exact power of two that brings a largest magnitude mx below 2^top (1 for a
+// zero or non-finite mx; the exponent is clamped so that the factor is finite)
+QCU_HD float pow2_scale(float mx, int top) {
+  if (!(mx > 0.f) || !(mx <= 3.4e38f)) return 1.f;
+  int e;
+  (void)frexpf(mx, &e);  // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
+  e = e < -110 ? -110 : e;
+  return ldexpf(1.f, top - e);
+}
+
+QCU_D h8v as_h8(uint4 u) { return __builtin_bit_cast(h8v, u); }
+
+// the two FP16 halves of a complex FP32 value: hi = (RN(re), RN(im)) packed,
+// lo = the rounded remainders.  The remainder x - hi is exact in FP32; one
+// v_fma_mix_f32 per component forms it from the packed FP16 operand directly.
+QCU_D void split2(float re, float im, unsigned &hi, unsigned &lo) {
+  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const h2v h = __builtin_convertvector((f2v){re, im}, h2v);
+  hi = __builtin_bit_cast(unsigned, h);
+  float rr, ri;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rr) : "v"(hi), "v"(re));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ri) : "v"(hi), "v"(im));
+  const h2v l = __builtin_convertvector((f2v){rr, ri}, h2v);
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+// Packs one layer's weights W[t][o][i][4x4] (complex, FP32) into the A
+// operands A[t][mt][ks][hl][lane] (8 FP16 each; hl 0 = hi, 1 = lo) of shape
+// NS, scaled by a power of two; winv[0] receives its inverse.  One block.
+template <int NS, int MT>
+__global__ void __launch_bounds__(256) k_net_pack_split(const float2 *W, int cin, int cout, uint4 *Apk, float *winv) {
+  constexpr int G = 64 / NS, KS = 8 / G;
+  __shared__ float red[256];
+  const int tid = threadIdx.x;
+  const int nw = 9 * cout * cin * 16;
+  float mx = 0.f;
+  for (int k = tid; k < nw; k += 256) mx = fmaxf(mx, fmaxf(fabsf(W[k].x), fabsf(W[k].y)));
+  red[tid] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmaxf(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  const float sw = pow2_scale(red[0], 14);
+  if (tid == 0) winv[0] = 1.f / sw;
+  const int total = 9 * MT * KS * 2 * 64;
+  for (int e = tid; e < total; e += 256) {
+    const int lane = e & 63;
+    int r = e >> 6;
+    const int hl = r & 1;
+    r >>= 1;
+    const int ks = r % KS;
+    r /= KS;
+    const int mt = r % MT, t = r / MT;
+    const int m = lane % NS;
+    const int o = (NS / 8) * mt + (m >> 3), s = (m >> 1) & 3, ri = m & 1;
+    const int i = G * ks + lane / NS;
+    h8v out;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int s2 = q >> 1, rin = q & 1;
+      float v = 0.f;
+      if (o < cout && i < cin) {
+        const float2 w = W[(((size_t)t * cout + o) * cin + i) * 16 + s * 4 + s2];
+        v = ri == 0 ? (rin == 0 ? w.x : -w.y) : (rin == 0 ? w.y : w.x);
+      }
+      v *= sw;
+      const _Float16 h = (_Float16)v;
+      out[q] = hl == 0 ? h : (_Float16)(v - (float)h);
+    }
+    Apk[e] = __builtin_bit_cast(uint4, out);
+  }
+}
+
+// v <- M v on the four spins of a colour-spin vector
+QCU_D void transport4(const cf (&m)[9], cf (&v)[4][3]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) transport1(m, v[s]);
+}
+
+// NS sites per wave, WAVES waves per block, MT M tiles (NS rows each: cout x 8 / NS)
+template <int NS, int WAVES, int MT>
+__global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) {
+  typedef SplitShape<NS> SH;
+  constexpr int G = 64 / NS, KS = 8 / G, NR = SH::NR;
+  constexpr int NA = 9 * MT * KS * 2 * 64;
+  __shared__ uint4 lw[NA];
+  if (a.st && a.st->done) return;  // uniform over the grid
+  for (int k = threadIdx.x; k < NA; k += blockDim.x) lw[k] = a.A[k];
+  __syncthreads();
+  const Geom g = a.g;
+  const float sin_ = pow2_scale(__uint_as_float(*a.in_max), 14);
+  const float oscale = a.winv[0] / sin_;
+  const int lane = threadIdx.x & 63, gq = lane / NS, n = lane % NS;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: chunk and parity in SGPRs
+  const int per_par = (g.vh + NS - 1) / NS;
+  const int nchunk = a.npar * per_par;
+  // chunk sequence of this wave: plain interleave over the grid, or (order 1)
+  // a contiguous eighth of the chunks per XCD (blocks are dealt to the 8 XCDs
+  // round robin), interleaved over the XCD's waves
+  // order 2: XCD x owns the z planes [x lz / 8, (x + 1) lz / 8) and its waves
+  // walk that slab in site order with t slowest, so the y, z and t neighbours
+  // a wave reads are read by waves of the same XCD (its L2) at about the same
+  // time; it needs lz % 8 == 0 and whole chunks per (z, t) plane
+  const int cpp = g.lxh * g.ly / NS;  // chunks per (z, t) plane
+  const bool slab = a.order == 2 && (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && cpp * NS == g.lxh * g.ly;
+  int c0, cend, cstep;
+  if (slab) {
+    const int nb = gridDim.x >> 3;
+    c0 = (int)(blockIdx.x >> 3) * WAVES + wave;
+    cend = a.npar * cpp * (g.lz >> 3) * g.lt;  // chunks of one slab
+    cstep = nb * WAVES;
+  } else if (a.order == 1 && (gridDim.x & 7) == 0) {
+    const int xcd = blockIdx.x & 7, nb = gridDim.x >> 3;
+    c0 = (int)((long long)nchunk * xcd / 8) + (int)(blockIdx.x >> 3) * WAVES + wave;
+    cend = (int)((long long)nchunk * (xcd + 1) / 8);
+    cstep = nb * WAVES;
+  } else {
+    c0 = blockIdx.x * WAVES + wave;
+    cend = nchunk;
+    cstep = gridDim.x * WAVES;
+  }
+  const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
+  const unsigned chan_off = (unsigned)(2 * field);  // one channel (both parities)
+  float omax = 0.f;
+#pragma unroll 1
+  for (int c = c0; c < cend; c += cstep) {
+    const int p = a.parity0 + (a.npar == 2 ? (c & 1) : 0);
+    int cs = a.npar == 2 ? c >> 1 : c;  // chunk in site order
+    if (slab) {
+      const int dz = g.lz >> 3, within = cs % cpp, q = cs / cpp;
+      cs = ((q / dz) * g.lz + (int)(blockIdx.x & 7) * dz + q % dz) * cpp + within;
+    }
+    const int base = cs * NS;
+    int i = base + n;
+    const bool ok = i < g.vh;
+    if (!ok) i = base;  // lanes past the last site compute a copy of a valid site and store nothing
+    const Site site = decode(g, i, p);
+    typename SH::acc_t acc[3][MT];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[cc][mt] = (typename SH::acc_t){};
+    // the lane's channel operand in K step ks: input channel G ks + gq at the
+    // term's neighbour; the base of channel G ks is uniform over the wave and
+    // the lane group adds gq channels (a 32-bit offset, checked at launch)
+    auto load_v = [&](float4 (&q)[6], const TermGeo &tg, int ks) {
+      const char *cb = a.in + ((size_t)(2 * G * ks) + tg.jp) * field;
+      const unsigned off = (unsigned)tg.j * 16u + (unsigned)gq * chan_off;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(cb, k * plane, off);
+    };
+    auto load_link = [&](const TermGeo &tg, float4 (&l)[3]) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
+    };
+    // one K step: the channel operand in q is unpacked and transported, then q
+    // is refilled with the next step's operand (next(q)) while the products run
+    auto step = [&](float4 (&q)[6], const cf (&m)[9], bool hop, int t, int ks, auto next) {
+      cf v[4][3];
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          const int ci = 3 * s2 + cc;
+          const float4 f = q[ci >> 1];
+          v[s2][cc] = (ci & 1) ? mk(f.z, f.w) : mk(f.x, f.y);
+        }
+      if (hop) {
+        transport4(m, v);  // m carries the input scale
+      } else {
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) v[s2][cc] = sin_ * v[s2][cc];
+      }
+      // the refill is issued here, a whole step ahead of its use: the
+      // scheduling barriers keep the compiler from sinking it below the
+      // products (where it would reuse the operand registers and leave the
+      // next step waiting on it)
+      __builtin_amdgcn_sched_barrier(0);
+      next(q);
+      __builtin_amdgcn_sched_barrier(0);
+      // the two FP16 halves of each colour's B operand
+      unsigned uh[3][4], ul[3][4];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) split2(v[s2][cc].re, v[s2][cc].im, uh[cc][s2], ul[cc][s2]);
+      // M tile by M tile: its two A halves, then three products per colour
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int ab = (((t * MT + mt) * KS + ks) * 2) * 64 + lane;
+        const h8v ah = as_h8(lw[ab]), al = as_h8(lw[ab + 64]);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          const h8v bh = as_h8(make_uint4(uh[cc][0], uh[cc][1], uh[cc][2], uh[cc][3]));
+          const h8v bl = as_h8(make_uint4(ul[cc][0], ul[cc][1], ul[cc][2], ul[cc][3]));
+          acc[cc][mt] = SH::mfma(ah, bh, acc[cc][mt]);
+          acc[cc][mt] = SH::mfma(ah, bl, acc[cc][mt]);
+          acc[cc][mt] = SH::mfma(al, bh, acc[cc][mt]);
+        }
+      }
+    };
+    // term outer, K step inner: each term's link is loaded and its third row
+    // rebuilt once, the next term's link while this one runs
+    TermGeo tg = term_geo(g, site, i, p, 0);
+    float4 lraw[3];
+    load_link(tg, lraw);
+    float4 vb[6];
+    load_v(vb, tg, 0);
+#pragma unroll 1
+    for (int t = 0; t < 9; ++t) {
+      cf m[9];
+      term_matrix(lraw, tg.hop, tg.fwd, tg.ph * sin_, m);
+      TermGeo ng = tg;
+      if (t + 1 < 9) {
+        ng = term_geo(g, site, i, p, t + 1);
+        if (ng.hop) load_link(ng, lraw);
+      }
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        step(vb, m, tg.hop, t, ks, [&](float4(&q)[6]) {
+          if (ks + 1 < KS) load_v(q, tg, ks + 1);
+          else if (t + 1 < 9) load_v(q, ng, 0);
+        });
+      }
+      tg = ng;
+    }
+    if (ok) {
+      // lane (n, gq) holds, per M tile and group of 4 registers j, output
+      // channel (NS / 8) mt + row(4 j, gq) / 8, spins sb and sb + 1 with
+      // sb = row(4 j, gq) % 8 / 2, all colours: float4 planes 3 sb / 2 .. + 2
+      const unsigned off = (unsigned)i * 16u;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int j = 0; j < NR / 4; ++j) {
+          const int r0 = SH::row(4 * j, gq);
+          const int o = (NS / 8) * mt + r0 / 8, sb = (r0 % 8) / 2;
+          char *ob = a.out + ((size_t)o * 2 + p) * field;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            // complex li = 2q, 2q + 1 of the lane's six: spin sb + li / 3, colour li % 3
+            const int l0 = 2 * q, l1 = 2 * q + 1;
+            const int e0 = 4 * j + 2 * (l0 / 3), e1 = 4 * j + 2 * (l1 / 3);
+            float4 f;
+            f.x = acc[l0 % 3][mt][e0] * oscale;
+            f.y = acc[l0 % 3][mt][e0 + 1] * oscale;
+            f.z = acc[l1 % 3][mt][e1] * oscale;
+            f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
+            omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+            st_at<float4>(ob, (size_t)(3 * (sb / 2) + q) * plane, off, f);
+          }
+        }
+    }
+  }
+  if (a.out_max) record_max(a.out_max, omax);
+}
+
+// MFMA shape of the split kernel: 32 sites per wave (v_mfma_f32_32x32x16_f16)
+// or 16 (v_mfma_f32_16x16x32_f16); QCU_NET_SPLIT_SHAPE selects it
+static int split_ns() {
+  static int ns = 0;
+  if (!ns) {
+    const char *e = getenv("QCU_NET_SPLIT_SHAPE");
+    ns = (e && atoi(e) == 16) ? 16 : 32;
+  }
+  return ns;
+}
+
+int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, hipStream_t s) {
+  QCU_CHECK(cin == 8 && (cout == 4 || cout == 8), QCU_ERR_ARG, "net: split-FP16 layer needs 8 -> 4 or 8 channels");
+  uint4 *A = (uint4 *)Apk;
+  if (split_ns() == 32) {
+    if (cout == 8) hipLaunchKernelGGL((k_net_pack_split<32, 2>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
+    else hipLaunchKernelGGL((k_net_pack_split<32, 1>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
+  } else {
+    if (cout == 8) hipLaunchKernelGGL((k_net_pack_split<16, 4>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
+    else hipLaunchKernelGGL((k_net_pack_split<16, 2>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
+  }
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s) {
+  QCU_CHECK(a.npar == 1 || a.npar == 2, QCU_ERR_ARG, "net: one or two output parities");
+  const int ns = split_ns();
+  static int wenv = -1;
+  if (wenv < 0) {
+    const char *e = getenv("QCU_NET_SPLIT_WAVES");
+    wenv = e ? atoi(e) : 0;
+  }
+  const int waves = (wenv == 4 || wenv == 8 || (wenv == 12 && ns == 16)) ? wenv : (ns == 32 ? 8 : 12);
+  // per-lane byte offsets within a K step's channels: (64 / NS - 1) x 192 + 6 x 16 bytes per site
+  QCU_CHECK((size_t)a.g.vh * ((64 / ns - 1) * 192 + 96) < ((size_t)1 << 32), QCU_ERR_DIMS,
+            "net: lattice too large for the split-FP16 layer");
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    QCU_HIP(hipGetDevice(&dev));
+    QCU_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu <= 0) ncu = 256;
+  }
+  const int nchunk = a.npar * ((a.g.vh + ns - 1) / ns);
+  // one block per CU (the weights fill its LDS); small lattices: no more blocks than chunks need
+  int nblk = nchunk < ncu * waves ? (nchunk + waves - 1) / waves : ncu;
+  if (a.order == 1 || a.order == 2) nblk = (nblk + 7) & ~7;
+  const dim3 grid(nblk), block(64 * waves);
+#define QCU_SPLIT_LAUNCH(NS_, W_)                                                          \
+  if (ns == NS_ && waves == W_) {                                                          \
+    if (cout == 8) hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 64 / NS_>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 32 / NS_>), grid, block, 0, s, a);           \
+  }
+  QCU_SPLIT_LAUNCH(32, 4) QCU_SPLIT_LAUNCH(32, 8) QCU_SPLIT_LAUNCH(16, 4) QCU_SPLIT_LAUNCH(16, 8)
+  QCU_SPLIT_LAUNCH(16, 12)
+#undef QCU_SPLIT_LAUNCH
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcu

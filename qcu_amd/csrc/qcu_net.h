@@ -1,0 +1,146 @@
+// qcu_net.h -- lattice geometry of the learned preconditioner's layers,
+// shared by its kernels: channel fields, the neighbour and link of each term,
+// and the colour transport.
+#pragma once
+#include "qcu_common.h"
+
+namespace qcu {
+
+typedef cplx<float> cf;
+
+// channel c, parity p of a channel-major activation block: one FP32 spinor
+// field of vh sites (96 bytes per site) per (channel, parity)
+QCU_D SpinorPtr chan_field(const SpinorPtr &base, int c, int p, int vh) {
+  SpinorPtr f;
+  f.p = (char *)base.p + ((size_t)c * 2 + p) * (size_t)vh * 96;
+  f.norm = nullptr;
+  f.stride = vh;
+  return f;
+}
+
+// neighbour site (half index) and link for term t at output site i of parity p
+struct TermGeo {
+  int j;      // input site (half index)
+  int jp;     // input parity
+  int hop;    // 0: site term
+  int fwd;
+  int slab;   // link slab (parity * 4 + mu)
+  int li;     // link site
+  float ph;
+};
+
+QCU_D TermGeo term_geo(const Geom &g, const Site &s, int i, int p, int t) {
+  TermGeo r;
+  r.j = i;
+  r.jp = p;
+  r.hop = t < 8;
+  r.fwd = !(t & 1);
+  r.slab = 0;
+  r.li = i;
+  r.ph = 1.f;
+  if (!r.hop) return r;
+  const int mu = t >> 1;
+  const int q = 1 - p;
+  const int sy = g.lxh, sz = g.lxh * g.ly, st_ = sz * g.lz;
+  int j = i;
+  if (mu == 0) {
+    const int xe = 1 - s.xodd;
+    j = r.fwd ? i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh : i - xe + xe * (int)(s.xh == 0) * g.lxh;
+  } else if (mu == 1) {
+    j = r.fwd ? i + sy - (int)(s.y == g.ly - 1) * g.ly * sy : i - sy + (int)(s.y == 0) * g.ly * sy;
+  } else if (mu == 2) {
+    j = r.fwd ? i + sz - (int)(s.z == g.lz - 1) * g.lz * sz : i - sz + (int)(s.z == 0) * g.lz * sz;
+  } else {
+    j = r.fwd ? i + st_ - (int)(s.t == g.lt - 1) * g.lt * st_ : i - st_ + (int)(s.t == 0) * g.lt * st_;
+    const bool cross = r.fwd ? (s.t == g.lt - 1) : (s.t == 0);
+    r.ph = (cross && g.tbc < 0) ? -1.f : 1.f;
+  }
+  r.j = j;
+  r.jp = q;
+  r.slab = r.fwd ? p * 4 + mu : q * 4 + mu;
+  r.li = r.fwd ? i : j;
+  return r;
+}
+
+// the colour matrix of a term, applied as v <- M v: M = ph U for a forward
+// hop and ph U^dag for a backward one, from the two stored rows of U; the
+// identity for the site term
+QCU_D void term_matrix(const float4 (&raw)[3], int hop, int fwd, float ph, cf (&m)[9]) {
+  if (!hop) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = mk(k % 4 == 0 ? 1.f : 0.f, 0.f);
+    return;
+  }
+  cf u[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u[2 * k] = mk(raw[k].x, raw[k].y);
+    u[2 * k + 1] = mk(raw[k].z, raw[k].w);
+  }
+  recon_row2(u);
+  if (fwd) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = mk(ph * u[k].re, ph * u[k].im);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[r * 3 + c] = mk(ph * u[c * 3 + r].re, -ph * u[c * 3 + r].im);
+  }
+}
+
+// v <- M v as a chain of fused multiply-adds
+QCU_D void transport1(const cf (&m)[9], cf (&v)[3]) {
+  cf a[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float re = m[r * 3].re * v[0].re, im = m[r * 3].re * v[0].im;
+    re = fmaf(-m[r * 3].im, v[0].im, re);
+    im = fmaf(m[r * 3].im, v[0].re, im);
+#pragma unroll
+    for (int c = 1; c < 3; ++c) {
+      re = fmaf(m[r * 3 + c].re, v[c].re, re);
+      re = fmaf(-m[r * 3 + c].im, v[c].im, re);
+      im = fmaf(m[r * 3 + c].re, v[c].im, im);
+      im = fmaf(m[r * 3 + c].im, v[c].re, im);
+    }
+    a[r] = mk(re, im);
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) v[r] = a[r];
+}
+
+struct SysState;
+
+// Hidden layer on the FP16 matrix cores (net_split.hip): 8 input channels,
+// 4 or 8 output channels, both in the channel-major FP32 layout.
+struct NetSplitArgs {
+  const char *in;           // input channel fields: (c * 2 + parity) * vh * 96 bytes
+  char *out;                // output channel fields, same layout
+  const uint4 *A;           // packed split weights (net_pack_split)
+  const float *winv;        // inverse of the weights' power-of-two scale
+  const unsigned *in_max;   // largest |component| of the input (float bits)
+  unsigned *out_max;        // receives the largest |component| of the output (or null)
+  LinkPtr U;                // FP32 links
+  Geom g;
+  int parity0, npar;        // output parities parity0 .. parity0 + npar - 1
+  int order;                // chunk order: 0 interleaved over the grid, 1 one eighth per XCD, 2 a z slab per XCD
+  const SysState *st;       // early-out flag of a captured solver iteration (or null)
+};
+int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, hipStream_t s);
+int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s);
+constexpr size_t kSplitPackBytes = (size_t)9 * 2 * 4 * 2 * 64 * 16;  // A operands of an 8 -> 8 layer
+
+// Records the largest |component| a wave wrote into *slot: the wave maximum,
+// then one atomic maximum on the float bits (non-negative floats order like
+// unsigned integers).  Every lane of the wave must call it.
+QCU_D void record_max(unsigned *slot, float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) {
+    const unsigned b = __float_as_uint(v);
+    if (b > *(volatile unsigned *)slot) atomicMax(slot, b);
+  }
+}
+
+}  // namespace qcu

@@ -193,6 +193,58 @@ def test_vector_layer_kernel_matches_oracle(orc, monkeypatch):
     assert e < 2e-5
 
 
+# Hidden 8 -> 8 and 8 -> 4 layers run on the FP16 matrix cores with split
+# operands (x = hi + lo, three FP16 products accumulated in FP32).  (6, 2, 2, 2)
+# has 24 sites per parity and (10, 6, 4, 4) 240: both end on a partial 32-site
+# chunk; order 1 deals one eighth of the chunks to each XCD.
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("L,chan", [((8, 4, 4, 8), (1, 8, 8, 8, 1)), ((6, 2, 2, 2), (1, 8, 8, 4, 1)),
+                                    ((10, 6, 4, 4), (1, 8, 8, 1))])
+def test_split_fp16_layers_match_oracle_and_fp32_kernel(orc, monkeypatch, L, chan, order):
+    """forward and input gradient with the split-FP16 hidden layers against the
+    oracle (FP32 bound 2e-5) and against the FP32 matrix-core kernel (2e-6)"""
+    tb = -1
+    monkeypatch.setenv("QCU_NET_SPLIT_ORDER", str(order))
+    U, lat, net, W = _setup(orc, L, chan, tb, 170)
+    monkeypatch.setenv("QCU_NET_SPLIT", "0")
+    from qcu_amd.lattice import LearnedPreconditioner
+    ref32 = LearnedPreconditioner(lat, chan, seed=170)
+    ref32.set_weights(W)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 171)
+    g = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 172)
+    ref, acts = orc.net_forward(x, W, chan, U, L, tb)
+    gin_ref, gW_ref = orc.net_backward(g, W, chan, U, L, acts, tb)
+    out = host(net.forward(dev(x))).ravel()
+    out32 = host(ref32.forward(dev(x))).ravel()
+    gW, gin = net.backward(dev(x), dev(g))
+    gW32, gin32 = ref32.backward(dev(x), dev(g))
+    gin, gin32 = host(gin).ravel(), host(gin32).ravel()
+    e = (rel(out, ref), rel(out, out32), rel(gin, gin_ref), rel(gin, gin32), rel(gW, gW_ref))
+    log(f"split L={L} chan={chan} order {order}: forward {e[0]:.1e} (FP32 kernel {e[1]:.1e}), "
+        f"input grad {e[2]:.1e} ({e[3]:.1e}), weight grad {e[4]:.1e}")
+    assert e[0] < 2e-5 and e[2] < 2e-5 and e[4] < 1e-4
+    assert e[1] < 2e-6 and e[3] < 2e-6
+    ref32.close()
+    net.close()
+
+
+@pytest.mark.parametrize("scale", [1e-25, 1e-8, 1e10, 1e25])
+def test_split_layers_follow_the_input_scale(orc, scale):
+    """K is linear: K(s x) / s = K(x) to FP32 accuracy for inputs far from unit
+    size (FP16 alone spans 6e-8 .. 6.5e4; the split kernel scales its operands
+    by powers of two from the largest component the previous layer wrote)"""
+    L, chan, tb = (8, 4, 4, 8), (1, 8, 8, 8, 1), -1
+    U, lat, net, W = _setup(orc, L, chan, tb, 180)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 181)
+    ref, _ = orc.net_forward(x, W, chan, U, L, tb)
+    out1 = host(net.forward(dev(x))).ravel()
+    outs = host(net.forward(dev(x * scale))).ravel() / scale
+    e_ref, e_one = rel(outs, ref), rel(outs, out1)
+    log(f"input scale {scale:g}: K(s x)/s vs oracle K(x) {e_ref:.1e}, vs K(x) {e_one:.1e}")
+    assert e_ref < 2e-5 and e_one < 2e-6
+    net.close()
+
+
 def test_replacing_the_network_recaptures_graphs(orc):
     """a solve with a new network (after the previous one was destroyed) must not
     replay graphs captured with the old one: graph and eager runs agree bit for bit"""
