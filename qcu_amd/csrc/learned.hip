@@ -1096,6 +1096,10 @@ struct SplitIO {
   const float *winv;        // inverse weight scale
   const unsigned *in_max;   // largest |component| of the input field
   unsigned *out_max;        // receives that of the output field
+  // fusion with the next, single-output layer (null A2: none)
+  const void *A2 = nullptr;
+  const float *winv2 = nullptr;
+  void *mix = nullptr;
 };
 
 struct Net : Precond {
@@ -1123,7 +1127,8 @@ struct Net : Precond {
   int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
   DevBuf spk;                   // packed operands: [direction][layer] x kSplitPackBytes
   DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
-  DevBuf swinv;                 // [direction][layer] inverse weight scale
+  DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
+  DevBuf fpk;                   // packed operands of the fused single-output layer
   unsigned long long wver = 1;  // weight version (bumped on every change of W)
   unsigned long long packed_ver = 0;
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
@@ -1144,6 +1149,14 @@ struct Net : Precond {
   }
   int pack_split(int dir, hipStream_t s);
   int reset_max(int dir, hipStream_t s);
+  // the last layer (8 -> 1 channels) can be folded into the epilogue of the
+  // 8 -> 8 layer before it when only the network output is needed
+  bool fusable() const {
+    return nl >= 2 && chan[nl] == 1 && chan[nl - 1] == 8 && chan[nl - 2] == 8 && split_layer(nl - 2, 0) &&
+           net_split_can_fuse() && mixb.p != nullptr;
+  }
+  // the second pass of the single-output layer: transports and sums the term fields in mixb
+  int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s);
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
   SpinorPtr act_ptr(int layer) const {
@@ -1157,8 +1170,9 @@ struct Net : Precond {
   // no maximum recorded)
   int layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout, int parity0, int nparity,
             const SysState *st, hipStream_t s, int zero_par = -1, const SplitIO *io = nullptr);
-  // zero_par0: parity on which the network input is zero (-1: none)
-  int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0);
+  // zero_par0: parity on which the network input is zero (-1: none);
+  // keep_acts false: only act[nl] is needed afterwards (the last layer may be fused)
+  int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts = true);
   // gout_zero_par: parity on which gout_last is zero; act0_zero_par: parity on
   // which the network input act[0] is zero (-1: none)
   int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1,
@@ -1246,7 +1260,8 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_TRY(Apk.alloc((size_t)9 * kNetMaxChan * 2 * kApkParts * 64 * 4));
   QCU_TRY(spk.alloc((size_t)2 * kNetMaxLayers * kSplitPackBytes));
   QCU_TRY(smax.alloc((size_t)2 * (kNetMaxLayers + 1) * 4));
-  QCU_TRY(swinv.alloc((size_t)2 * kNetMaxLayers * 4));
+  QCU_TRY(swinv.alloc((size_t)(2 * kNetMaxLayers + 1) * 4));
+  QCU_TRY(fpk.alloc(kFusePackBytes));
   QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
@@ -1298,6 +1313,9 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
     b.npar = nparity;
     b.order = split_order;
     b.st = st;
+    b.A2 = (const uint4 *)io->A2;
+    b.winv2 = io->winv2;
+    b.mix = (char *)io->mix;
     return net_layer_split(b, cout, s);
   }
   // one output channel: the input channels are mixed per term on the matrix
@@ -1325,16 +1343,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
       else hipLaunchKernelGGL((k_net_mix<0, false>), g1, dim3(256), 0, s, m);
     }
     QCU_HIP(hipGetLastError());
-    NetGatherArgs ga;
-    ga.mix = m.mix;
-    ga.out = out;
-    ga.U = a.U;
-    ga.g = lat->g;
-    ga.parity0 = parity0;
-    ga.st = st;
-    hipLaunchKernelGGL(k_net_gather, dim3((unsigned)ndiv(vh, 256), nparity), dim3(256), 0, s, ga);
-    QCU_HIP(hipGetLastError());
-    return 0;
+    return gather(out, parity0, nparity, st, s);
   }
   if (use_mfma && cout > 1) {
     const int mtiles = (cout + 3) / 4;
@@ -1393,6 +1402,21 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
 }
 
 // act[0] holds the input; fills act[1..nl] (act[nl] = output)
+int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s) {
+  NetGatherArgs ga;
+  ga.mix.p = mixb.p;
+  ga.mix.norm = nullptr;
+  ga.mix.stride = lat->g.vh;
+  ga.out = out;
+  ga.U = lat->links(P32);
+  ga.g = lat->g;
+  ga.parity0 = parity0;
+  ga.st = st;
+  hipLaunchKernelGGL(k_net_gather, dim3((unsigned)ndiv(lat->g.vh, 256), nparity), dim3(256), 0, s, ga);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
 // Packs the split operands of every split layer in direction dir (0: the
 // weights, 1: their transposes in Wt) and records the weight version packed.
 int Net::pack_split(int dir, hipStream_t s) {
@@ -1403,6 +1427,8 @@ int Net::pack_split(int dir, hipStream_t s) {
     QCU_TRY(net_pack_split(w, dir ? chan[k + 1] : chan[k], dir ? chan[k] : chan[k + 1], (void *)io.A,
                            (float *)io.winv, s));
   }
+  if (dir == 0 && fusable())
+    QCU_TRY(net_pack_fuse((const float2 *)W.p + woff[nl - 1], fpk.p, (float *)swinv.p + 2 * kNetMaxLayers, s));
   if (dir == 0) packed_ver = wver;
   return 0;
 }
@@ -1420,20 +1446,33 @@ int Net::reset_max(int dir, hipStream_t s) {
   return 0;
 }
 
-int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0) {
+int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts) {
   bool any = false;
   for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
   if (any) {
     if (packed_ver != wver) QCU_TRY(pack_split(0, s));
     QCU_TRY(reset_max(0, s));
   }
+  // fused: the 8 -> 8 layer before the last writes the last layer's mixed
+  // term fields, so act[nl - 1] is never written and the last layer is only
+  // its gather pass
+  const bool fuse = !keep_acts && odd_only_last && fusable();
   for (int k = 0; k < nl; ++k) {
     const bool last = k == nl - 1;
     const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
+    if (fuse && last) {
+      QCU_TRY(gather(act_ptr(k + 1), p0, np, st, s));
+      break;
+    }
     // the kernel before a split layer records the largest component it writes
     SplitIO io = split_io(k, 0);
     if (!split_layer(k, 0)) io.A = nullptr;
     if (!(k + 1 < nl && split_layer(k + 1, 0))) io.out_max = nullptr;
+    if (fuse && k == nl - 2) {
+      io.A2 = fpk.p;
+      io.winv2 = (const float *)swinv.p + 2 * kNetMaxLayers;
+      io.mix = mixb.p;
+    }
     QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s,
                   k == 0 ? zero_par0 : -1, any ? &io : nullptr));
   }
@@ -1530,7 +1569,7 @@ int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
     default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
   }
   QCU_HIP(hipGetLastError());
-  QCU_TRY(forward(st, S.stream, true, 0));
+  QCU_TRY(forward(st, S.stream, true, 0, false));
   const SpinorPtr an = act_ptr(nl);
   switch (prec) {
     case P64: hipLaunchKernelGGL(k_net_extract<P64>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
@@ -1839,7 +1878,7 @@ int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
   QCU_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
   QCU_HIP(hipGetLastError());
-  QCU_TRY(net.forward(nullptr, s, true, 0));
+  QCU_TRY(net.forward(nullptr, s, true, 0, false));
   SpinorPtr o1 = net.act_ptr(net.nl);
   o1.p = (char *)o1.p + (size_t)vh * 96;  // channel 0, parity 1
   hipLaunchKernelGGL(k_store_odd, dim3(nb), dim3(bs), 0, s, (double2 *)out, o1, vh);

@@ -153,16 +153,139 @@ __global__ void __launch_bounds__(256) k_net_pack_split(const float2 *W, int cin
   }
 }
 
+// Packs the single-output layer's weights W'[t][0][i][4x4] as the A operands
+// of the fused epilogue (32-site shape): tile sel 0, 1 hold hop terms 4 sel ..
+// 4 sel + 3 (rows m = 8 tl + 2 s + ri), tile 2 the site term (rows 0..7; the
+// rest zero); K step ks covers input channels 2 ks, 2 ks + 1 in the order the
+// epilogue holds them, k = 8 g + 4 cl + 2 sl + ri: channel 2 ks + cl, spin
+// 2 g + sl.  A[sel][ks][hl][lane], 8 FP16 each; winv[0] = inverse scale.
+__global__ void __launch_bounds__(256) k_net_pack_fuse(const float2 *W, uint4 *Apk, float *winv) {
+  __shared__ float red[256];
+  const int tid = threadIdx.x;
+  const int nw = 9 * 8 * 16;
+  float mx = 0.f;
+  for (int k = tid; k < nw; k += 256) mx = fmaxf(mx, fmaxf(fabsf(W[k].x), fabsf(W[k].y)));
+  red[tid] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmaxf(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  const float sw = pow2_scale(red[0], 14);
+  if (tid == 0) winv[0] = 1.f / sw;
+  for (int e = tid; e < 3 * 4 * 2 * 64; e += 256) {
+    const int lane = e & 63;
+    int r = e >> 6;
+    const int hl = r & 1;
+    r >>= 1;
+    const int ks = r & 3, sel = r >> 2;
+    const int m = lane & 31, g = lane >> 5;
+    const int tl = m >> 3, s = (m >> 1) & 3, ri = m & 1;
+    const int t = sel < 2 ? 4 * sel + tl : (tl == 0 ? 8 : -1);
+    h8v out;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int cl = q >> 2, sl = (q >> 1) & 1, rin = q & 1;
+      const int i = 2 * ks + cl, s2 = 2 * g + sl;
+      float v = 0.f;
+      if (t >= 0) {
+        const float2 w = W[((size_t)t * 8 + i) * 16 + s * 4 + s2];
+        v = ri == 0 ? (rin == 0 ? w.x : -w.y) : (rin == 0 ? w.y : w.x);
+      }
+      v *= sw;
+      const _Float16 h = (_Float16)v;
+      out[q] = hl == 0 ? h : (_Float16)(v - (float)h);
+    }
+    Apk[e] = __builtin_bit_cast(uint4, out);
+  }
+}
+
 // v <- M v on the four spins of a colour-spin vector
 QCU_D void transport4(const cf (&m)[9], cf (&v)[4][3]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) transport1(m, v[s]);
 }
 
+// Epilogue of an 8 -> 8 layer fused with the next, single-output layer
+// (32-site shape).  Lane (n, g) holds spins 2g, 2g + 1 of all 8 output
+// channels of site n: exactly the K rows 8 g .. 8 g + 7 of each K step in the
+// order k = 8 g + 4 cl + 2 sl + ri (channel 2 ks + cl, spin 2 g + sl), so
+// the accumulators are the B operand of the mixing Y_t = W'_t out without
+// any exchange between lanes.  The site's values get a power-of-two scale
+// from their largest component (the two lanes of a site share it) before the
+// FP16 split.  Even output sites produce the 8 hop-term fields (two M tiles),
+// odd ones the site-term field (one tile, rows 0..7).
+QCU_D void fused_epilogue(const NetSplitArgs &a, const f16v (&acc)[3][2], float oscale, int i, int p, bool ok,
+                          int g, int lane, size_t field, size_t plane) {
+  float mx = 0.f;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) mx = fmaxf(mx, fabsf(acc[cc][mt][r]));
+  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  const float sn = pow2_scale(mx * oscale, 14);
+  const float f = oscale * sn;                 // accumulator -> scaled B operand (a power of two)
+  const float yscale = a.winv2[0] / sn;        // mixed field -> its true value
+  const unsigned off = (unsigned)i * 16u;
+  const int nsel = p == 0 ? 2 : 1;
+#pragma unroll 1
+  for (int k = 0; k < nsel; ++k) {
+    const int sel = p == 0 ? k : 2;
+    f16v dy[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) dy[cc] = (f16v){};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int ab = ((sel * 4 + ks) * 2) * 64 + lane;
+      const h8v ah = as_h8(a.A2[ab]), al = as_h8(a.A2[ab + 64]);
+      const int mt = ks >> 1;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        unsigned uh[4], ul[4];
+#pragma unroll
+        for (int cl = 0; cl < 2; ++cl)
+#pragma unroll
+          for (int sl = 0; sl < 2; ++sl) {
+            const int r = 4 * (2 * (ks & 1) + cl) + 2 * sl;
+            split2(acc[cc][mt][r] * f, acc[cc][mt][r + 1] * f, uh[2 * cl + sl], ul[2 * cl + sl]);
+          }
+        const h8v bh = as_h8(make_uint4(uh[0], uh[1], uh[2], uh[3]));
+        const h8v bl = as_h8(make_uint4(ul[0], ul[1], ul[2], ul[3]));
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, dy[cc], 0, 0, 0);
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, dy[cc], 0, 0, 0);
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, dy[cc], 0, 0, 0);
+      }
+    }
+    if (!ok) continue;
+    // register r of dy: row 8 (r / 4) + 4 g + r % 4 = term 4 sel + r / 4 (the
+    // site term: r / 4 == 0), spin 2 g + (r % 4) / 2
+#pragma unroll
+    for (int tl = 0; tl < 4; ++tl) {
+      if (sel == 2 && tl > 0) break;
+      const int t = sel == 2 ? 8 : 4 * sel + tl;
+      char *ob = a.mix + ((size_t)t * 2 + p) * field;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int l0 = 2 * q, l1 = 2 * q + 1;
+        const int e0 = 4 * tl + 2 * (l0 / 3), e1 = 4 * tl + 2 * (l1 / 3);
+        float4 v;
+        v.x = dy[l0 % 3][e0] * yscale;
+        v.y = dy[l0 % 3][e0 + 1] * yscale;
+        v.z = dy[l1 % 3][e1] * yscale;
+        v.w = dy[l1 % 3][e1 + 1] * yscale;
+        st_at<float4>(ob, (size_t)(3 * g + q) * plane, off, v);
+      }
+    }
+  }
+}
+
 // NS sites per wave, WAVES waves per block, MT M tiles (NS rows each: cout x 8 / NS)
-template <int NS, int WAVES, int MT>
+template <int NS, int WAVES, int MT, bool FUSE = false>
 __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) {
   typedef SplitShape<NS> SH;
+  static_assert(!FUSE || (NS == 32 && MT == 2), "the fused epilogue is written for the 32-site shape, 8 outputs");
   constexpr int G = 64 / NS, KS = 8 / G, NR = SH::NR;
   constexpr int NA = 9 * MT * KS * 2 * 64;
   __shared__ uint4 lw[NA];
@@ -309,6 +432,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
       }
       tg = ng;
     }
+    if constexpr (FUSE) {
+      fused_epilogue(a, acc, oscale, i, p, ok, gq, lane, field, plane);
+      continue;
+    }
     if (ok) {
       // lane (n, gq) holds, per M tile and group of 4 registers j, output
       // channel (NS / 8) mt + row(4 j, gq) / 8, spins sb and sb + 1 with
@@ -365,6 +492,14 @@ int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, h
   return 0;
 }
 
+bool net_split_can_fuse() { return split_ns() == 32; }
+
+int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_pack_fuse, dim3(1), dim3(256), 0, s, W, (uint4 *)Apk, winv);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
 int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s) {
   QCU_CHECK(a.npar == 1 || a.npar == 2, QCU_ERR_ARG, "net: one or two output parities");
   const int ns = split_ns();
@@ -394,8 +529,14 @@ int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s) {
     if (cout == 8) hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 64 / NS_>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 32 / NS_>), grid, block, 0, s, a);           \
   }
-  QCU_SPLIT_LAUNCH(32, 4) QCU_SPLIT_LAUNCH(32, 8) QCU_SPLIT_LAUNCH(16, 4) QCU_SPLIT_LAUNCH(16, 8)
-  QCU_SPLIT_LAUNCH(16, 12)
+  if (a.A2) {
+    QCU_CHECK(ns == 32 && cout == 8 && a.mix && a.winv2, QCU_ERR_ARG, "net: fused layer needs the 32-site shape");
+    if (waves == 4) hipLaunchKernelGGL((k_net_layer_split<32, 4, 2, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_net_layer_split<32, 8, 2, true>), grid, block, 0, s, a);
+  } else {
+    QCU_SPLIT_LAUNCH(32, 4) QCU_SPLIT_LAUNCH(32, 8) QCU_SPLIT_LAUNCH(16, 4) QCU_SPLIT_LAUNCH(16, 8)
+    QCU_SPLIT_LAUNCH(16, 12)
+  }
 #undef QCU_SPLIT_LAUNCH
   QCU_HIP(hipGetLastError());
   return 0;

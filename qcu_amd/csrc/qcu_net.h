@@ -126,10 +126,24 @@ struct NetSplitArgs {
   int parity0, npar;        // output parities parity0 .. parity0 + npar - 1
   int order;                // chunk order: 0 interleaved over the grid, 1 one eighth per XCD, 2 a z slab per XCD
   const SysState *st;       // early-out flag of a captured solver iteration (or null)
+  // fused single-output layer (8 -> 8 layers only, when the next layer maps 8
+  // channels to 1 and only its mixed term fields are needed): instead of its
+  // output channels the kernel writes, per output site, the next layer's
+  // mixed term fields Y_t = W'_t out (hop terms at even sites, the site term
+  // at odd sites) into mix, in the channel layout (term t, parity q at slot
+  // (t, q)); null A2: no fusion
+  const uint4 *A2;          // packed W' (net_pack_fuse)
+  const float *winv2;       // its inverse scale
+  char *mix;
 };
 int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, hipStream_t s);
 int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s);
+// the 8 -> 1 weights W'[t][0][i][4x4] as the fused epilogue's A operands
+int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s);
+// whether net_layer_split can fuse the next single-output layer (32-site shape)
+bool net_split_can_fuse();
 constexpr size_t kSplitPackBytes = (size_t)9 * 2 * 4 * 2 * 64 * 16;  // A operands of an 8 -> 8 layer
+constexpr size_t kFusePackBytes = (size_t)3 * 4 * 2 * 64 * 16;       // those of the fused 8 -> 1 mixing
 
 // Records the largest |component| a wave wrote into *slot: the wave maximum,
 // then one atomic maximum on the float bits (non-negative floats order like

@@ -228,6 +228,36 @@ def test_split_fp16_layers_match_oracle_and_fp32_kernel(orc, monkeypatch, L, cha
     net.close()
 
 
+# The preconditioner application folds the last layer (8 -> 1) into the
+# epilogue of the 8 -> 8 layer before it: that kernel writes the last layer's
+# mixed term fields instead of its own output channels.
+@pytest.mark.parametrize("L,chan", [((8, 4, 4, 8), (1, 8, 8, 8, 1)), ((6, 2, 2, 2), (1, 8, 8, 1)),
+                                    ((10, 6, 4, 4), (1, 8, 8, 8, 1))])
+def test_fused_last_layer_apply_odd(orc, monkeypatch, L, chan):
+    """apply_odd with the fused last layer against the oracle (FP32 bound 2e-5)
+    and against the unfused FP32 matrix-core path (2e-6)"""
+    U, lat, net, W = _setup(orc, L, chan, -1, 190)
+    monkeypatch.setenv("QCU_NET_SPLIT", "0")
+    from qcu_amd.lattice import LearnedPreconditioner
+    ref32 = LearnedPreconditioner(lat, chan, seed=190)
+    ref32.set_weights(W)
+    xo = orc.gauss((orc.vol(L) // 2, 4, 3), 191)
+    full = np.zeros((1, 2) + xo.shape, complex)
+    full[0, 1] = xo
+    ref, _ = orc.net_forward(full, W, chan, U, L, -1)
+    ref = ref.reshape(2, -1)[1]
+    got = host(net.apply_odd(dev(xo))).ravel()
+    got32 = host(ref32.apply_odd(dev(xo))).ravel()
+    # the full forward (all activations kept, no fusion) agrees too
+    fwd = host(net.forward(dev(full))).reshape(2, -1)[1]
+    e = (rel(got, ref), rel(got, got32), rel(fwd, ref))
+    log(f"fused apply_odd L={L} chan={chan}: vs oracle {e[0]:.1e}, vs FP32 kernels {e[1]:.1e}, "
+        f"unfused forward vs oracle {e[2]:.1e}")
+    assert e[0] < 2e-5 and e[1] < 2e-6 and e[2] < 2e-5
+    ref32.close()
+    net.close()
+
+
 @pytest.mark.parametrize("scale", [1e-25, 1e-8, 1e10, 1e25])
 def test_split_layers_follow_the_input_scale(orc, scale):
     """K is linear: K(s x) / s = K(x) to FP32 accuracy for inputs far from unit
