@@ -28,6 +28,7 @@ EPS, CSW, KAPPA, TB = 0.2, 1.0, 0.1248, -1
 TOL = 1e-10
 PEAK_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP16_TFLOPS = 2500.0
 # algorithmic bytes per output site of one hop launch (8 neighbour spinors,
 # 8 compressed links, 1 output) and of the packed clover inverse, per precision
 HOP_BYTES = {0: 8 * 192 + 8 * 96 + 192, 1: 8 * 96 + 8 * 48 + 96, 2: 8 * 52 + 8 * 24 + 52}
@@ -35,7 +36,7 @@ CLOVER_BYTES = {0: 576, 1: 288, 2: 148}
 # the xpay epilogue of the Schur operator reads one more spinor
 XPAY_BYTES = {0: 192, 1: 96, 2: 52}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp16_clover_32x32x32x64_traffic.json")
-NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_32x32x32x64_traffic.json")
+NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_split_32x32x32x64_traffic.json")
 HOP64_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_clover_32x32x32x64_traffic.json")
 
 
@@ -191,10 +192,20 @@ def main():
     log(f"learned+mixed: {t_learned * 1e3:.1f} ms {lres[-1]}")
 
     # roofline of the dominant kernel of the timed solve: the network's 8 -> 8
-    # channel layer (FP32 matrix cores, compute-bound), timed with HIP events
+    # layer on the FP16 matrix cores with split FP32 operands (k_net_layer_split),
+    # timed with HIP events on the stream it runs on.  Its bound is memory: the
+    # compulsory bytes are the 8 input channels (both parities), the 8 output
+    # channels and the FP32 links (each link serves two sites); the measured
+    # traffic (profiles/) shows the neighbour channels re-read past L2.
     ms_layer, fl_layer = net.bench_layer(1, iters=10)
-    tflops = fl_layer / (ms_layer * 1e-3) / 1e12
+    layer_bytes = 8 * 2 * vh * 96 * 2 + 8 * vh * 48
+    layer_GBps = layer_bytes / (ms_layer * 1e-3) / 1e9
     net_traffic = traffic_of(NET_TRAFFIC)
+    # executed matrix-core work: three FP16 products of the real-embedded
+    # complex mixing, 9 terms x 8 x 8 channels x 16 spin pairs x 3 colours x
+    # 4 real multiply-adds x 2 flops per site
+    mfma_flops = 3 * 9 * 8 * 8 * 16 * 3 * 4 * 2 * 2 * vh
+    ms_apply = net.bench_apply(iters=10)
     # the inner-precision clover hop with the xpay epilogue (HBM-bound)
     ms_hop = lat.bench_hop(prec=inner, clover=True, xpay=True, iters=100)
     alg = (HOP_BYTES[inner] + CLOVER_BYTES[inner] + XPAY_BYTES[inner]) * vh
@@ -249,14 +260,19 @@ def main():
                      "pair_generation_seconds": round(pair_s, 2),
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
-        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_mfma<2, 8>)", "bound": "mfma",
-                     "compute": "FP32 MFMA v_mfma_f32_16x16x4_f32, three real products per complex product",
-                     "flop_count": "6 flops per complex multiply-add in the channel mixing (as executed), "
-                                   "8 in the colour transports",
-                     "achieved": round(tflops, 2),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_TFLOPS, 3),
-                     "traffic": net_traffic,
-                     "ms_per_launch": round(ms_layer, 4), "alg_flops_per_launch": fl_layer},
+        "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_split<32, 8, 2>)", "bound": "hbm",
+                     "compute": "v_mfma_f32_32x32x16_f16, FP32 operands split into FP16 hi + lo, three products "
+                                "accumulated in FP32 (FP32-accurate)",
+                     "bytes_count": "compulsory: 8 input channels + 8 output channels (FP32, both parities) "
+                                    "+ FP32 links once",
+                     "achieved": round(layer_GBps, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(layer_GBps / PEAK_GBS, 3), "traffic": net_traffic,
+                     "traffic_GBps": None if net_traffic is None else round(net_traffic / (ms_layer * 1e-3) / 1e9, 1),
+                     "ms_per_launch": round(ms_layer, 4), "alg_bytes_per_launch": layer_bytes,
+                     "fp32_equivalent_TFLOPs": round(fl_layer / (ms_layer * 1e-3) / 1e12, 1),
+                     "fp16_mfma_TFLOPs": round(mfma_flops / (ms_layer * 1e-3) / 1e12, 1),
+                     "fp16_mfma_frac": round(mfma_flops / (ms_layer * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 3)},
+        "net_apply_ms": round(ms_apply, 4),
         "hop_roofline": {"kernel": f"hop {args.inner} clover xpay", "bound": "hbm",
                          "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / PEAK_GBS, 3), "traffic": traffic,

@@ -215,6 +215,10 @@ int qcuNetLoss(QcuNetHandle h, const void *const *srcs, int n, double *loss);
  * the flops count 8 per complex multiply-add, 6 in the matrix-core channel
  * mixing (three real products per complex product) */
 int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *flops);
+/* average ms of one preconditioner application as the solver runs it (odd
+ * field embedded, forward pass keeping only the output, odd part extracted;
+ * no counterpart in the reference) */
+int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms);
 /* out = K in on one-parity (odd) external fields, as applied inside the solver */
 int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);
 /* weights file: one line "QCUNET1 {json metadata}" (channels, weight count,

@@ -98,6 +98,7 @@ SIGNATURES = {
     "qcuNetApplyOdd": (C.c_int, [_VP, _VP, _VP]),
     "qcuNetLossGrad": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
     "qcuNetBenchLayer": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "qcuNetBenchApply": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "qcuNetSave": (C.c_int, [_VP, C.c_char_p]),
     "qcuNetLoad": (C.c_int, [_VP, C.c_char_p]),
 }

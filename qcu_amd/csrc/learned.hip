@@ -1125,6 +1125,7 @@ struct Net : Precond {
   // on the FP32 matrix-core kernel)
   bool use_split = true;
   int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
+  int split_ns = 32;            // sites per wave of the split kernel (QCU_NET_SPLIT_SHAPE)
   DevBuf spk;                   // packed operands: [direction][layer] x kSplitPackBytes
   DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
   DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
@@ -1153,7 +1154,7 @@ struct Net : Precond {
   // 8 -> 8 layer before it when only the network output is needed
   bool fusable() const {
     return nl >= 2 && chan[nl] == 1 && chan[nl - 1] == 8 && chan[nl - 2] == 8 && split_layer(nl - 2, 0) &&
-           net_split_can_fuse() && mixb.p != nullptr;
+           net_split_can_fuse(split_ns) && mixb.p != nullptr;
   }
   // the second pass of the single-output layer: transports and sums the term fields in mixb
   int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s);
@@ -1265,6 +1266,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
+  split_ns = net_split_shape_default();
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
   // a layer with one output and several input channels (the last layer, and
@@ -1311,6 +1313,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
     b.g = lat->g;
     b.parity0 = parity0;
     b.npar = nparity;
+    b.ns = split_ns;
     b.order = split_order;
     b.st = st;
     b.A2 = (const uint4 *)io->A2;
@@ -1425,7 +1428,7 @@ int Net::pack_split(int dir, hipStream_t s) {
     const SplitIO io = split_io(k, dir);
     const float2 *w = (const float2 *)(dir ? Wt.p : W.p) + woff[k];
     QCU_TRY(net_pack_split(w, dir ? chan[k + 1] : chan[k], dir ? chan[k] : chan[k + 1], (void *)io.A,
-                           (float *)io.winv, s));
+                           (float *)io.winv, split_ns, s));
   }
   if (dir == 0 && fusable())
     QCU_TRY(net_pack_fuse((const float2 *)W.p + woff[nl - 1], fpk.p, (float *)swinv.p + 2 * kNetMaxLayers, s));
@@ -1692,6 +1695,17 @@ int net_precond(void *handle, const Lattice *lat, Precond **out) {
   return 0;
 }
 
+// uniform values in [-1, 1) from a hash of the index
+__global__ void k_fill_hash(float *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 2654435761u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  p[i] = (float)(h >> 8) * (2.f / 16777216.f) - 1.f;
+}
+
 // the network runs on its lattice's resident FP32 links: a lattice whose gauge
 // (re)load failed has none
 static int net_ready(const Net &net) {
@@ -1952,6 +1966,47 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
     const double per_site = mix + 8.0 * (8.0 * (two_pass ? 1 : ci) * 4 * 9);
     *flops = per_site * 2.0 * net.lat->g.vh;
   }
+  return 0;
+}
+
+// average ms of one preconditioner application as the solver runs it (odd
+// FP32 field embedded, the forward pass with only the output kept, odd part
+// extracted), on internal buffers, timed with HIP events on the lattice stream
+int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms) {
+  clear_error();
+  QCU_CHECK(h && ms && iters > 0, QCU_ERR_ARG, "bad arguments");
+  Net &net = h->net;
+  QCU_TRY(net_ready(net));
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  NetTrainer tr{net, s};
+  auto once = [&]() {
+    hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
+    QCU_HIP(hipGetLastError());
+    QCU_TRY(net.forward(nullptr, s, true, 0, false));
+    hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, tr.odd(1), net.act_ptr(net.nl), vh, nullptr);
+    QCU_HIP(hipGetLastError());
+    return 0;
+  };
+  // a deterministic input of unit size (all-zero operands would run the
+  // matrix cores at an unrepresentative clock)
+  const size_t nf = (size_t)vh * 24;
+  hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)ndiv(nf, 256)), dim3(256), 0, s, (float *)tr.odd(0).p, nf);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(once());
+  hipEvent_t e0, e1;
+  QCU_HIP(hipEventCreate(&e0));
+  QCU_HIP(hipEventCreate(&e1));
+  QCU_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < iters; ++k) QCU_TRY(once());
+  QCU_HIP(hipEventRecord(e1, s));
+  QCU_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  QCU_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / iters;
   return 0;
 }
 

@@ -282,7 +282,7 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const f16v (&acc)[3][2], float 
 }
 
 // NS sites per wave, WAVES waves per block, MT M tiles (NS rows each: cout x 8 / NS)
-template <int NS, int WAVES, int MT, bool FUSE = false>
+template <int NS, int WAVES, int MT, bool FUSE = false, bool DIAG = false>
 __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) {
   typedef SplitShape<NS> SH;
   static_assert(!FUSE || (NS == 32 && MT == 2), "the fused epilogue is written for the 32-site shape, 8 outputs");
@@ -307,7 +307,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   // a wave reads are read by waves of the same XCD (its L2) at about the same
   // time; it needs lz % 8 == 0 and whole chunks per (z, t) plane
   const int cpp = g.lxh * g.ly / NS;  // chunks per (z, t) plane
-  const bool slab = a.order == 2 && (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && cpp * NS == g.lxh * g.ly;
+  // order 3 (diagonal): a chunk is one x row (NS == lxh), XCD x owns the
+  // planes [x lz / 8, (x + 1) lz / 8) in pairs and walks each pair with t
+  // slowest, both parities of a row side by side; every wave takes its terms
+  // in an order rotated by phi = (y + 2 z + 3 t) mod 7 (see the term loop), so
+  // that all the waves reading one input row read it in the same slot
+  const bool diag = DIAG && a.order == 3 && NS == g.lxh && a.npar == 2 && (gridDim.x & 7) == 0 && (g.lz & 15) == 0;
+  const bool slab = (a.order == 2 || diag) && (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && cpp * NS == g.lxh * g.ly;
   int c0, cend, cstep;
   if (slab) {
     const int nb = gridDim.x >> 3;
@@ -331,7 +337,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   for (int c = c0; c < cend; c += cstep) {
     const int p = a.parity0 + (a.npar == 2 ? (c & 1) : 0);
     int cs = a.npar == 2 ? c >> 1 : c;  // chunk in site order
-    if (slab) {
+    int phi = 0;
+    if (diag) {
+      int r1 = cs;  // y fastest, then the plane within the pair, t, the pair
+      const int y = r1 % g.ly;
+      r1 /= g.ly;
+      const int zz = r1 & 1;
+      r1 >>= 1;
+      const int tt = r1 % g.lt, pair = r1 / g.lt;
+      const int z = (int)(blockIdx.x & 7) * (g.lz >> 3) + 2 * pair + zz;
+      cs = (tt * g.lz + z) * g.ly + y;
+      phi = (y + 2 * z + 3 * tt) % 7;
+    } else if (slab) {
       const int dz = g.lz >> 3, within = cs % cpp, q = cs / cpp;
       cs = ((q / dz) * g.lz + (int)(blockIdx.x & 7) * dz + q % dz) * cpp + within;
     }
@@ -407,6 +424,49 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
         }
       }
     };
+    if constexpr (DIAG) {
+      // (diag holds: the launcher picks this instance only when it does)
+      // Diagonal schedule.  Slot sigma holds: 0 the local terms (x+, site,
+      // x-), then y- 1, z- 2, t- 3, t+ 4, z+ 5, y+ 6, i.e. +-a_mu for the
+      // backward / forward hop with a = (y 1, z 2, t 3).  At step k a wave
+      // takes slot (k + phi) mod 7: a wave at R reads R + e_mu in the forward
+      // slot -a_mu, so the reader of input row S through any term takes it in
+      // step -phi(S) mod 7.  K step outer, so that one slot is one step.
+      constexpr unsigned long long slot_terms[7] = {0x108, 0x3, 0x5, 0x7, 0x6, 0x4, 0x2};
+      constexpr int slot_len[7] = {3, 1, 1, 1, 1, 1, 1};
+      unsigned long long seq = 0;  // 9 terms, 4 bits each
+      int nseq = 0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int sg = (k + phi) % 7;
+        seq |= slot_terms[sg] << (4 * nseq);
+        nseq += slot_len[sg];
+      }
+      auto term_at = [&](int j) { return (int)((seq >> (4 * j)) & 15u); };
+      TermGeo tg = term_geo(g, site, i, p, term_at(0));
+      float4 lraw[3];
+      if (tg.hop) load_link(tg, lraw);
+      float4 vb[6];
+      load_v(vb, tg, 0);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll 1
+        for (int j = 0; j < 9; ++j) {
+          const int t = term_at(j);
+          cf m[9];
+          term_matrix(lraw, tg.hop, tg.fwd, tg.ph * sin_, m);
+          const bool last = ks + 1 == KS && j == 8;
+          const TermGeo ng = term_geo(g, site, i, p, term_at(j == 8 ? 0 : j + 1));
+          step(vb, m, tg.hop, t, ks, [&](float4(&q)[6]) {
+            if (!last) {
+              if (ng.hop) load_link(ng, lraw);
+              load_v(q, ng, j == 8 ? ks + 1 : ks);
+            }
+          });
+          tg = ng;
+        }
+      }
+    } else {
     // term outer, K step inner: each term's link is loaded and its third row
     // rebuilt once, the next term's link while this one runs
     TermGeo tg = term_geo(g, site, i, p, 0);
@@ -431,6 +491,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
         });
       }
       tg = ng;
+    }
     }
     if constexpr (FUSE) {
       fused_epilogue(a, acc, oscale, i, p, ok, gq, lane, field, plane);
@@ -468,20 +529,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
 }
 
 // MFMA shape of the split kernel: 32 sites per wave (v_mfma_f32_32x32x16_f16)
-// or 16 (v_mfma_f32_16x16x32_f16); QCU_NET_SPLIT_SHAPE selects it
-static int split_ns() {
-  static int ns = 0;
-  if (!ns) {
-    const char *e = getenv("QCU_NET_SPLIT_SHAPE");
-    ns = (e && atoi(e) == 16) ? 16 : 32;
-  }
-  return ns;
+// or 16 (v_mfma_f32_16x16x32_f16); QCU_NET_SPLIT_SHAPE=16 selects the second
+// (read when a network is created: its packed weights follow the shape)
+int net_split_shape_default() {
+  const char *e = getenv("QCU_NET_SPLIT_SHAPE");
+  return (e && atoi(e) == 16) ? 16 : 32;
 }
 
-int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, hipStream_t s) {
+int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, int ns, hipStream_t s) {
   QCU_CHECK(cin == 8 && (cout == 4 || cout == 8), QCU_ERR_ARG, "net: split-FP16 layer needs 8 -> 4 or 8 channels");
+  QCU_CHECK(ns == 16 || ns == 32, QCU_ERR_ARG, "net: split shape 16 or 32");
   uint4 *A = (uint4 *)Apk;
-  if (split_ns() == 32) {
+  if (ns == 32) {
     if (cout == 8) hipLaunchKernelGGL((k_net_pack_split<32, 2>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
     else hipLaunchKernelGGL((k_net_pack_split<32, 1>), dim3(1), dim3(256), 0, s, W, cin, cout, A, winv);
   } else {
@@ -492,7 +551,7 @@ int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, h
   return 0;
 }
 
-bool net_split_can_fuse() { return split_ns() == 32; }
+bool net_split_can_fuse(int ns) { return ns == 32; }
 
 int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s) {
   hipLaunchKernelGGL(k_net_pack_fuse, dim3(1), dim3(256), 0, s, W, (uint4 *)Apk, winv);
@@ -502,7 +561,8 @@ int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s) {
 
 int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s) {
   QCU_CHECK(a.npar == 1 || a.npar == 2, QCU_ERR_ARG, "net: one or two output parities");
-  const int ns = split_ns();
+  const int ns = a.ns;
+  QCU_CHECK(ns == 16 || ns == 32, QCU_ERR_ARG, "net: split shape 16 or 32");
   static int wenv = -1;
   if (wenv < 0) {
     const char *e = getenv("QCU_NET_SPLIT_WAVES");
@@ -522,14 +582,20 @@ int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s) {
   const int nchunk = a.npar * ((a.g.vh + ns - 1) / ns);
   // one block per CU (the weights fill its LDS); small lattices: no more blocks than chunks need
   int nblk = nchunk < ncu * waves ? (nchunk + waves - 1) / waves : ncu;
-  if (a.order == 1 || a.order == 2) nblk = (nblk + 7) & ~7;
+  if (a.order >= 1 && a.order <= 3) nblk = (nblk + 7) & ~7;
   const dim3 grid(nblk), block(64 * waves);
 #define QCU_SPLIT_LAUNCH(NS_, W_)                                                          \
   if (ns == NS_ && waves == W_) {                                                          \
     if (cout == 8) hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 64 / NS_>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_net_layer_split<NS_, W_, 32 / NS_>), grid, block, 0, s, a);           \
   }
-  if (a.A2) {
+  const bool diag = a.order == 3 && ns == 16 && a.g.lxh == 16 && a.npar == 2 && (a.g.lz & 15) == 0 && cout == 8 &&
+                    !a.A2;
+  if (diag) {
+    const int nb8 = (a.g.vh * 2 / 16 + 8 * 8 - 1) / (8 * 8);  // blocks of 8 waves, at most one per CU
+    const dim3 g8((unsigned)(((nb8 < ncu ? nb8 : ncu) + 7) & ~7)), b8(64 * 8);
+    hipLaunchKernelGGL((k_net_layer_split<16, 8, 4, false, true>), g8, b8, 0, s, a);
+  } else if (a.A2) {
     QCU_CHECK(ns == 32 && cout == 8 && a.mix && a.winv2, QCU_ERR_ARG, "net: fused layer needs the 32-site shape");
     if (waves == 4) hipLaunchKernelGGL((k_net_layer_split<32, 4, 2, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_net_layer_split<32, 8, 2, true>), grid, block, 0, s, a);

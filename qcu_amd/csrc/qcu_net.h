@@ -124,7 +124,9 @@ struct NetSplitArgs {
   LinkPtr U;                // FP32 links
   Geom g;
   int parity0, npar;        // output parities parity0 .. parity0 + npar - 1
-  int order;                // chunk order: 0 interleaved over the grid, 1 one eighth per XCD, 2 a z slab per XCD
+  int ns;                   // sites per wave: 32 (v_mfma_f32_32x32x16_f16) or 16 (16x16x32), as packed
+  int order;                // chunk order: 0 interleaved over the grid, 1 one eighth per XCD, 2 a z slab per
+                            // XCD, 3 z-plane pairs per XCD with the diagonal term schedule (16 sites, Lx = 32)
   const SysState *st;       // early-out flag of a captured solver iteration (or null)
   // fused single-output layer (8 -> 8 layers only, when the next layer maps 8
   // channels to 1 and only its mixed term fields are needed): instead of its
@@ -136,12 +138,13 @@ struct NetSplitArgs {
   const float *winv2;       // its inverse scale
   char *mix;
 };
-int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, hipStream_t s);
+int net_pack_split(const float2 *W, int cin, int cout, void *Apk, float *winv, int ns, hipStream_t s);
+int net_split_shape_default();
 int net_layer_split(const NetSplitArgs &a, int cout, hipStream_t s);
 // the 8 -> 1 weights W'[t][0][i][4x4] as the fused epilogue's A operands
 int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s);
 // whether net_layer_split can fuse the next single-output layer (32-site shape)
-bool net_split_can_fuse();
+bool net_split_can_fuse(int ns);
 constexpr size_t kSplitPackBytes = (size_t)9 * 2 * 4 * 2 * 64 * 16;  // A operands of an 8 -> 8 layer
 constexpr size_t kFusePackBytes = (size_t)3 * 4 * 2 * 64 * 16;       // those of the fused 8 -> 1 mixing
 

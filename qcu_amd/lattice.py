@@ -462,6 +462,12 @@ class LearnedPreconditioner:
         _lib.check(self._lib.qcuNetBenchLayer(self.handle, int(layer), int(iters), C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
 
+    def bench_apply(self, iters=10):
+        """ms per preconditioner application as the solver runs it (HIP events)"""
+        ms = C.c_double()
+        _lib.check(self._lib.qcuNetBenchApply(self.handle, int(iters), C.byref(ms)))
+        return ms.value
+
     def loss(self, sources) -> float:
         n = len(sources)
         sp = (C.c_void_p * n)(*[_field_ptr(s, self.lattice.n_half, "source").value for s in sources])

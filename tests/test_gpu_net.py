@@ -228,6 +228,35 @@ def test_split_fp16_layers_match_oracle_and_fp32_kernel(orc, monkeypatch, L, cha
     net.close()
 
 
+# Both MFMA shapes of the split kernel (32 sites per wave on 32x32x16, 16 on
+# 16x16x32) and its chunk orders: 2 walks a z slab per XCD; 3 (16 sites, Lx
+# 32, Lz a multiple of 16) gives each wave one x row and rotates its term
+# order by (y + 2 z + 3 t) mod 7.
+@pytest.mark.parametrize("shape,order", [(32, 2), (16, 0), (16, 2), (16, 3)])
+def test_split_shapes_and_orders(orc, monkeypatch, shape, order):
+    """forward and input gradient of a 1-8-8-8-1 network on 32x2x16x2 against
+    the oracle (2e-5) for each split shape and chunk order"""
+    monkeypatch.setenv("QCU_NET_SPLIT_SHAPE", str(shape))
+    monkeypatch.setenv("QCU_NET_SPLIT_ORDER", str(order))
+    L, chan, tb = (32, 2, 16, 2), (1, 8, 8, 8, 1), -1
+    U, lat, net, W = _setup(orc, L, chan, tb, 200)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 201)
+    g = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 202)
+    ref, acts = orc.net_forward(x, W, chan, U, L, tb)
+    gin_ref, _ = orc.net_backward(g, W, chan, U, L, acts, tb)
+    out = host(net.forward(dev(x))).ravel()
+    _, gin = net.backward(dev(x), dev(g))
+    xo = x[0, 1]
+    full = np.zeros_like(x)
+    full[0, 1] = xo
+    ref_odd, _ = orc.net_forward(full, W, chan, U, L, tb)
+    got_odd = host(net.apply_odd(dev(np.ascontiguousarray(xo)))).ravel()
+    e = (rel(out, ref), rel(host(gin).ravel(), gin_ref), rel(got_odd, ref_odd.reshape(2, -1)[1]))
+    log(f"split shape {shape} order {order}: forward {e[0]:.1e}, input grad {e[1]:.1e}, apply_odd {e[2]:.1e}")
+    assert max(e) < 2e-5
+    net.close()
+
+
 # The preconditioner application folds the last layer (8 -> 1) into the
 # epilogue of the 8 -> 8 layer before it: that kernel writes the last layer's
 # mixed term fields instead of its own output channels.
