@@ -100,6 +100,16 @@ int qcuGaussian(void *field, long n_complex, uint64_t seed);
 /* out = [A_p^-1] D_{p,1-p} in on external one-parity fields; clover = 1 adds
  * the inverse clover of the lattice; prec selects the kernel precision */
 int qcuDslash(QcuLatticeHandle h, void *out, const void *in, int parity, int clover, int prec);
+/* qcuDslash without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as qcuDslash.  The two staged half spinors live in a 32 MiB device
+ * array of the library (one per GPU, held until the process ends), so the
+ * lattice must satisfy 2 x Vh x (192 bytes FP64, 96 FP32, 52 FP16, each
+ * rounded up to 256 bytes) <= 32 MiB (e.g. 16^3x32 in FP64); larger ones are
+ * refused with QCU_ERR_DIMS.  Calls that overlap on one GPU share that array
+ * and race; the call is synchronous, like qcuDslash. */
+int qcuDslashNoAlloc(QcuLatticeHandle h, void *out, const void *in, int parity, int clover, int prec);
 /* out = M in = (A - kappa D) in on the full lattice, FP64 */
 int qcuMatFull(QcuLatticeHandle h, void *out, const void *in);
 /* |b - M x| / |b| in FP64 on the full lattice */

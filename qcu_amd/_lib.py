@@ -64,6 +64,7 @@ SIGNATURES = {
     "qcuGaugeRandom": (C.c_int, [_VP, _I4, C.c_double, C.c_uint64]),
     "qcuGaussian": (C.c_int, [_VP, C.c_long, C.c_uint64]),
     "qcuDslash": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int]),
+    "qcuDslashNoAlloc": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int]),
     "qcuMatFull": (C.c_int, [_VP, _VP, _VP]),
     "qcuTrueResidual": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_double)]),
     "qcuSolve": (C.c_int, [_VP, _VP, _VP, C.POINTER(SolveParam), C.POINTER(SolveInfo)]),

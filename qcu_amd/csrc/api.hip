@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 
 #include "../../include/qcu.h"
 #include "qcu_lattice.h"
@@ -294,6 +295,79 @@ int qcuDslash(QcuLatticeHandle h, void *out, const void *in, int parity, int clo
   QCU_TRY(hop(lat.self, prec, b.ptr(), a.ptr(), parity, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr,
               tuned_cfg(lat.self, prec), lat.stream, clover != 0));
   QCU_TRY(spinor_batch_to_ext(b.ptr(), lat.self, 0, 0, out, prec, lat.stream));
+  QCU_HIP(hipStreamSynchronize(lat.stream));
+  return 0;
+}
+
+// ---- the hop without allocation ----------------------------------------------
+// Staging of qcuDslashNoAlloc: the input and the output half spinor in the
+// internal layout.  A file-scope device array: every GPU the library runs on
+// holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device
+// address); it uses the address hipGetSymbolAddress returns for the current
+// device, fetched once per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static int noalloc_scratch(int dev, void **ptr) {
+  QCU_CHECK(dev >= 0 && dev < kNoAllocMaxDevices, QCU_ERR_STATE, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    QCU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QCU_CHECK(p != nullptr, QCU_ERR_STATE, "no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
+
+static size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// bytes of one staged half spinor (the FP16 one carries its per-site scales)
+static size_t noalloc_half_bytes(int prec, int vh) {
+  return round256(spinor_bytes(prec, vh)) + (prec == P16 ? round256((size_t)vh * sizeof(float)) : 0);
+}
+
+int qcuDslashNoAlloc(QcuLatticeHandle h, void *out, const void *in, int parity, int clover, int prec) {
+  clear_error();
+  QCU_CHECK(h, QCU_ERR_ARG, "null handle");
+  Lattice &lat = h->lat;
+  QCU_CHECK(prec >= 0 && prec <= 2, QCU_ERR_ARG, "bad precision");
+  const size_t half = noalloc_half_bytes(prec, lat.g.vh);
+  QCU_CHECK(2 * half <= kNoAllocScratchBytes, QCU_ERR_DIMS,
+            "qcuDslashNoAlloc: the two staged half spinors need " + std::to_string(2 * half) +
+                " bytes, above the " + std::to_string(kNoAllocScratchBytes) + "-byte scratch array");
+  QCU_CHECK(lat.gauge_loaded, QCU_ERR_STATE, "gauge field not loaded");
+  QCU_CHECK(parity == 0 || parity == 1, QCU_ERR_ARG, "parity must be 0 or 1");
+  QCU_CHECK((lat.prec_mask >> prec) & 1, QCU_ERR_ARG, "precision not resident");
+  QCU_CHECK(!clover || lat.has_clover(), QCU_ERR_STATE, "lattice has no clover term (csw = 0)");
+  QCU_TRY(check_dev_ptr(out, "out"));
+  QCU_TRY(check_dev_ptr(in, "in"));
+  int dev = -1;
+  QCU_HIP(hipGetDevice(&dev));
+  QCU_CHECK(dev == lat.device, QCU_ERR_STATE, "the current device is not the lattice's");
+  void *base = nullptr;
+  QCU_TRY(noalloc_scratch(dev, &base));
+  SpinorPtr a, b;
+  const size_t data = round256(spinor_bytes(prec, lat.g.vh));
+  a.p = base;
+  a.norm = prec == P16 ? (float *)((char *)base + data) : nullptr;
+  a.stride = lat.g.vh;
+  b.p = (char *)base + half;
+  b.norm = prec == P16 ? (float *)((char *)base + half + data) : nullptr;
+  b.stride = lat.g.vh;
+  QCU_TRY(spinor_ext_to_batch(in, lat.self, 0, 0, a, prec, lat.stream));
+  SpinorPtr none{};
+  QCU_TRY(hop(lat.self, prec, b, a, parity, EPI_PLAIN, none, RED_NONE, none, nullptr, nullptr,
+              tuned_cfg(lat.self, prec), lat.stream, clover != 0));
+  QCU_TRY(spinor_batch_to_ext(b, lat.self, 0, 0, out, prec, lat.stream));
   QCU_HIP(hipStreamSynchronize(lat.stream));
   return 0;
 }
