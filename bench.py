@@ -111,12 +111,15 @@ def main():
     ap.add_argument("--inner", default="fp16", choices=["fp32", "fp16"])
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update residual drop")
     ap.add_argument("--train-sources", type=int, default=256)
-    ap.add_argument("--train-epochs", type=int, default=1)
+    ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--train-batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--solution-weight", type=float, default=0.0,
                     help="weight of the (source, solution) pair loss |K c - y|^2/|y|^2 in training "
                          "(0: the residual loss |Mhat K c - c|^2/|c|^2 only)")
+    ap.add_argument("--pairs", action="store_true",
+                    help="generate the (source, solution) pairs with the plain FP64 solver (counted in the "
+                         "training cost) even when the loss does not use the solutions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -164,19 +167,22 @@ def main():
     srcs = [gaussian((vh, 4, 3), 10000 + k) for k in range(args.train_sources)]
     torch.cuda.synchronize()
     pair_s = 0.0
-    if args.solution_weight > 0:
-        # (source, solution) pairs from the mixed-precision solver, y = Mhat^-1 c
+    sols = None
+    if args.solution_weight > 0 or args.pairs:
+        # (source, solution) pairs from the plain FP64 solver, y = Mhat^-1 c
         t0 = time.perf_counter()
-        sols = schur_solutions(lat, srcs, tol=1e-8, prec=inner, reliable_delta=args.delta)
+        sols = schur_solutions(lat, srcs, tol=1e-8, prec=FP64)
         torch.cuda.synchronize()
         pair_s = time.perf_counter() - t0
+    if args.solution_weight > 0:
         losses, train_s = net.train_pairs(srcs, sols, epochs=args.train_epochs, batch=args.train_batch,
                                           lr=args.lr, solution_weight=args.solution_weight)
-        del sols
     else:
         losses, train_s = net.train(srcs, epochs=args.train_epochs, batch=args.train_batch, lr=args.lr)
+    del sols
     del srcs
     log(f"training: {train_s:.1f} s, epoch losses {losses.tolist()}")
+    sol_pairs = args.solution_weight > 0 or args.pairs
     learned = lambda: lat.solve(b, tol=TOL, prec=inner, reliable_delta=args.delta, precond=net)
     for _ in range(args.warmup):
         learned()
@@ -239,7 +245,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (outer) / " + args.inner + " (inner) / f32 (network)",
         "data": "synthetic: U = exp(i 0.2 H) seeded, Gaussian source; network trained here on "
-                f"{args.train_sources} Gaussian sources",
+                f"{args.train_sources} Gaussian sources"
+                + (" with their plain FP64 solutions" if sol_pairs else ""),
         "config": {"workload": "32^3x64 clover, eps 0.2, csw 1.0, kappa 0.1248, antiperiodic t, "
                                "tol 1e-10 true residual",
                    "fast_path": f"learned preconditioner (4 layers, 8 channels, FP32) + mixed "
@@ -258,6 +265,7 @@ def main():
                      "loss": "residual |Mhat K c - c|^2/|c|^2" if args.solution_weight <= 0 else
                              f"{args.solution_weight} x pair |K c - y|^2/|y|^2 + {1 - args.solution_weight:g} x residual",
                      "pair_generation_seconds": round(pair_s, 2),
+                     "pairs": "plain FP64 BiCGSTAB to 1e-8" if sol_pairs else None,
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
         "roofline": {"kernel": "network layer 8->8 channels (k_net_layer_split<32, 8, 2>)", "bound": "hbm",

@@ -275,7 +275,8 @@ def net_apply(args):
     x = gaussian((lat.vh, 4, 3), 7)
     for _ in range(3):
         net.apply_odd(x)
-    return {"L": lat.L, "net_apply_ms_median": _median_ms(lambda: net.apply_odd(x), args.iters)}
+    return {"L": lat.L, "net_apply_ms_median": _median_ms(lambda: net.apply_odd(x), args.iters),
+            "net_apply_ms_events": net.bench_apply(iters=args.iters)}
 
 
 def net_backward(args):
