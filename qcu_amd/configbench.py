@@ -244,6 +244,10 @@ def _median_ms(fn, n):
 
 def hop(args):
     lat = _headline_lattice(args)
+    if args.shape:
+        lat.set_launch_shape(args.prec, *args.shape)
+    elif args.tune:
+        lat.tune(args.prec)
     ms = lat.bench_hop(prec=args.prec, clover=bool(args.clover), xpay=bool(args.xpay), iters=args.iters)
     alg = (HOP[args.prec] + (CL[args.prec] if args.clover else 0) + (XP[args.prec] if args.xpay else 0)) * lat.vh
     return {"L": lat.L, "prec": args.prec, "clover": args.clover, "xpay": args.xpay,
@@ -326,6 +330,9 @@ def main():
     ap.add_argument("--xpay", type=int, default=1)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--layer", type=int, default=1)
+    ap.add_argument("--tune", type=int, default=1, help="hop: tune the launch shape first (0: default shape)")
+    ap.add_argument("--shape", type=int, nargs=3, metavar=("BLOCK", "REMAP", "GROUP"),
+                    help="hop: run this launch shape (as the tuner would pick it) instead of tuning")
     ap.add_argument("--mib", type=int, default=1024)
     args = ap.parse_args()
     for w in args.which:

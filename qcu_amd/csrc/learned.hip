@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/qcu.h"
@@ -607,6 +608,10 @@ struct NetWgradArgs {
   float *partials;  // [combo][block][CO * 32]
   int act_zero;     // parity on which the input is known to be zero (-1: none)
   int gout_zero;    // parity on which the output gradient is known to be zero (-1: none)
+  // largest |component| of the input and of the output gradient (float bits),
+  // recorded by the kernels that wrote them: the split-FP16 products scale
+  // their operands by the powers of two these give
+  const unsigned *amax = nullptr, *gmax = nullptr;
 };
 
 template <int CO>
@@ -673,18 +678,32 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // Weight gradient on the matrix cores.  For term t,
 //   gW[t][o][i][s][s2] = sum over sites x of both parities and colours c of
 //                        g_o(x)[s][c] conj(v_i(x)[s2][c]),  v_i = T_t a_i,
-// i.e. D[m = (o, s)][n = (i, s2)] = sum_k G[m][k] conj(V)[k][n] over k = (x, c),
-// as three real FP32 products per colour on v_mfma_f32_16x16x4_f32 (the
-// three-multiplication complex product, with G conj(V)):
-//   P1 += Gr Vr,  P2 += Gi Vi,  P3 += (Gr + Gi)(Vr - Vi),
-//   Re D = P1 + P2,  Im D = P3 - P1 + P2
-// (accumulators dr = P1, dn = P2, di = P3).
-// A K step is 4 sites of one colour: lane l (group q = l / 16, index n = l % 16)
-// feeds A[m = n][k = q] = g of row (o = 4 mt + n / 4, s = n % 4) at site q of
-// the group, and B[k = q][n] = v of column (i = 4 nt + n / 4, s2 = n % 4) at the
-// same site; it ends holding D rows (o = 4 mt + q, s = j), column n.  Each wave
-// sweeps groups of 4 sites; the 4 waves of a block are summed through LDS and
-// the block sums are added in a fixed order by k_net_wgrad_fin.
+// i.e. D[m = (o, s)][n = (i, s2)] = sum_k G[m][k] conj(V)[k][n] over k = (x, c).
+// Two forms of the products:
+// * FP32 (SPLIT false): three real products per colour on
+//   v_mfma_f32_16x16x4_f32 (the three-multiplication complex product, with
+//   G conj(V)):
+//     P1 += Gr Vr,  P2 += Gi Vi,  P3 += (Gr + Gi)(Vr - Vi),
+//     Re D = P1 + P2,  Im D = P3 - P1 + P2
+//   (accumulators dr = P1, dn = P2, di = P3).  A K step is 4 sites of one
+//   colour: lane l (group q = l / 16, index n = l % 16) feeds A[m = n][k = q] =
+//   g of row (o = 4 mt + n / 4, s = n % 4) at site q of the group, and
+//   B[k = q][n] = v of column (i = 4 nt + n / 4, s2 = n % 4) at the same site.
+// * split FP16 (SPLIT true; the 8 -> 8 layers, whose two operand fields have
+//   recorded maxima): v_mfma_f32_16x16x32_f16 with K step = the 3 colours of 4
+//   sites as real pairs, lane group q supplying k = 8 q .. 8 q + 7 = (Re, Im)
+//   of colours 0, 1, 2 of site q and two zeros.  With A = (Gr, Gi) pairs,
+//     Re D = A (Vr, Vi),  Im D = A (-Vi, Vr)
+//   (G conj V = Gr Vr + Gi Vi + i (Gi Vr - Gr Vi)), each product of FP32
+//   operands split as x = hi + lo in three FP16 products (Ootomo & Yokota,
+//   IJHPCA 36 (2022); lo x lo dropped), after scaling G and V by powers of two
+//   from their fields' largest components (G below 2^14; V, a transported
+//   colour vector, below sqrt(3) 2^14).  Accumulators dr = Re D, di = Im D in
+//   the scaled units; k_net_wgrad_fin divides the scales out in FP64.
+// Either way lane (q, n) ends holding D rows (o = 4 mt + q, s = j), column n.
+// Each wave sweeps groups of 4 sites; the 4 waves of a block are summed
+// through LDS and the block sums are added in a fixed order by
+// k_net_wgrad_fin.
 //
 // The grid is one row of 9 x kWgBlocks blocks.  The 9 blocks (one per term)
 // of a site block sb are 8 block indices apart inside a window of 72: blocks
@@ -695,13 +714,14 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // works on one contiguous stretch of sites (their y neighbours included).
 constexpr int kWgBlocks = 1024;  // blocks per term (a multiple of 8)
 
-template <int MT, int NT>
+template <int MT, int NT, bool SPLIT>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
   const Geom g = a.g;
   const int b = blockIdx.x;
   const int nsb = gridDim.x / 9;
   const int t = (b >> 3) % 9, sb = (b & 7) * (nsb >> 3) + b / 72;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int q = lane >> 4, n = lane & 15;
   // output parities this term needs: a nonzero output gradient there, and an
   // input (parity 1 - p for a hop, p for the site term) not known to be zero;
@@ -717,15 +737,18 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   const int nw = nsb * 4;
   const size_t vh = (size_t)g.vh;
   const int rs = n & 3, rslot = (3 * rs) >> 1, rodd = (3 * rs) & 1;  // spin of this lane's row / column
-  f4v dr[MT][NT], di[MT][NT], dn[MT][NT];
+  f4v dr[MT][NT], di[MT][NT], dn[SPLIT ? 1 : MT][SPLIT ? 1 : NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
       dr[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
       di[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
-      dn[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
+      if (!SPLIT) dn[mt][nt] = (f4v){0.f, 0.f, 0.f, 0.f};
     }
+  // operand scales of the split products (powers of two, exact)
+  const float sg = SPLIT ? pow2_scale(__uint_as_float(*a.gmax), 14) : 1.f;
+  const float sv = SPLIT ? pow2_scale(__uint_as_float(*a.amax), 14) : 1.f;
   // the lane's spin (3 colours) out of its two raw float4 slots
   auto spin3 = [&](const float4 (&r)[2], float keep, cf (&v)[3]) {
     const float4 x = r[0], y = r[1];
@@ -753,9 +776,10 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   const int o_n = n >> 2;
   float grow[MT], gcol[NT];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) grow[mt] = 4 * mt + o_n < a.cout ? 1.f : 0.f;
+  for (int mt = 0; mt < MT; ++mt) grow[mt] = 4 * mt + o_n < a.cout ? sg : 0.f;
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) gcol[nt] = 4 * nt + o_n < a.cin ? 1.f : 0.f;
+  for (int nt = 0; nt < NT; ++nt) gcol[nt] = 4 * nt + o_n < a.cin ? sv : 0.f;
+  const TermConst tcon = term_const(g, t);
   auto geo = [&](int grp) {
     Geo r;
     int k = grp * 4 + q;
@@ -765,9 +789,11 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     r.p = k >= g.vh ? 1 : 0;
     r.i = k - r.p * g.vh;
     r.keep = ok ? 1.f : 0.f;
-    r.tg = term_geo(g, decode(g, r.i, r.p), r.i, r.p, t);
+    r.tg = term_geo_u(g, decode(g, r.i, r.p), r.i, r.p, tcon);
     return r;
   };
+  // every term loads a link (the site term a valid one it does not apply), so
+  // that the loads of the sweep below sit behind no branch
   auto load = [&](Ops &x, const Geo &ge) {
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
@@ -783,11 +809,9 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       x.v[nt][0] = b[0];
       x.v[nt][1] = b[vh];
     }
-    if (ge.tg.hop) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        x.l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
-    }
+    for (int k = 0; k < 3; ++k)
+      x.l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
   };
   auto compute = [&](const Ops &x, const Geo &ge) {
     // rows: output gradient of channels 4 mt + n / 4, spin n % 4
@@ -796,12 +820,50 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     for (int mt = 0; mt < MT; ++mt) spin3(x.g[mt], ge.keep * grow[mt], gv[mt]);
     // columns: transported inputs of channels 4 nt + n / 4, spin n % 4
     cf u[9];
-    if (ge.tg.hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
+    const bool hop = tcon.hop;  // uniform
+    if (hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
     cf vv[NT][3];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
       spin3(x.v[nt], gcol[nt], vv[nt]);
-      if (ge.tg.hop) transport1(u, vv[nt]);
+      if (hop) transport1(u, vv[nt]);
+    }
+    if (SPLIT) {
+      uint4 ah[MT], al[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        split2(gv[mt][0].re, gv[mt][0].im, ah[mt].x, al[mt].x);
+        split2(gv[mt][1].re, gv[mt][1].im, ah[mt].y, al[mt].y);
+        split2(gv[mt][2].re, gv[mt][2].im, ah[mt].z, al[mt].z);
+        ah[mt].w = 0u;
+        al[mt].w = 0u;
+      }
+      // (Re, Im) -> (-Im, Re) on a packed FP16 pair
+      auto rot = [](unsigned u) { return __builtin_amdgcn_alignbit(u, u, 16) ^ 0x8000u; };
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        uint4 bh, bl;
+        split2(vv[nt][0].re, vv[nt][0].im, bh.x, bl.x);
+        split2(vv[nt][1].re, vv[nt][1].im, bh.y, bl.y);
+        split2(vv[nt][2].re, vv[nt][2].im, bh.z, bl.z);
+        bh.w = 0u;
+        bl.w = 0u;
+        const uint4 ch = make_uint4(rot(bh.x), rot(bh.y), rot(bh.z), 0u);
+        const uint4 cl = make_uint4(rot(bl.x), rot(bl.y), rot(bl.z), 0u);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          f4v r = dr[mt][nt], i = di[mt][nt];
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(ah[mt]), as_h8(bh), r, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(ah[mt]), as_h8(ch), i, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(ah[mt]), as_h8(bl), r, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(ah[mt]), as_h8(cl), i, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(al[mt]), as_h8(bh), r, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(al[mt]), as_h8(ch), i, 0, 0, 0);
+          dr[mt][nt] = r;
+          di[mt][nt] = i;
+        }
+      }
+      return;
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -811,7 +873,8 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
         for (int nt = 0; nt < NT; ++nt) {
           const float ar = gv[mt][c].re, ai = gv[mt][c].im, br = vv[nt][c].re, bi = vv[nt][c].im;
           dr[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, dr[mt][nt], 0, 0, 0);
-          dn[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dn[mt][nt], 0, 0, 0);
+          dn[SPLIT ? 0 : mt][SPLIT ? 0 : nt] =
+              __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, dn[SPLIT ? 0 : mt][SPLIT ? 0 : nt], 0, 0, 0);
           di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br - bi, di[mt][nt], 0, 0, 0);
         }
   };
@@ -825,26 +888,24 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     load(A, gA);
 #pragma unroll 1
     while (true) {
-      const int g1 = grp + nw;
-      const bool m1 = g1 < ngroups;
-      gB = geo(m1 ? g1 : grp);
+      const bool m1 = grp + nw < ngroups;
+      gB = geo(m1 ? grp + nw : grp);
       load(B, gB);
       __builtin_amdgcn_sched_barrier(0);
       compute(A, gA);
       if (!m1) break;
-      grp = g1;
-      const int g2 = grp + nw;
-      const bool m2 = g2 < ngroups;
-      gA = geo(m2 ? g2 : grp);
+      grp += nw;
+      const bool m2 = grp + nw < ngroups;
+      gA = geo(m2 ? grp + nw : grp);
       load(A, gA);
       __builtin_amdgcn_sched_barrier(0);
       compute(B, gB);
       if (!m2) break;
-      grp = g2;
+      grp += nw;
     }
   }
-  // block sum in a fixed order: registers (mt, nt, part, j) of every lane
   constexpr int NREG = MT * NT * 2 * 4;
+  // block sum in a fixed order: registers (mt, nt, part, j) of every lane
   __shared__ float sh[4][NREG][64];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
@@ -853,8 +914,14 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = ((mt * NT + nt) * 2) * 4 + j;
-        sh[wave][r][lane] = dr[mt][nt][j] + dn[mt][nt][j];
-        sh[wave][r + 4][lane] = di[mt][nt][j] - dr[mt][nt][j] + dn[mt][nt][j];
+        if (SPLIT) {
+          sh[wave][r][lane] = dr[mt][nt][j];
+          sh[wave][r + 4][lane] = di[mt][nt][j];
+        } else {
+          const float p2 = dn[SPLIT ? 0 : mt][SPLIT ? 0 : nt][j];
+          sh[wave][r][lane] = dr[mt][nt][j] + p2;
+          sh[wave][r + 4][lane] = di[mt][nt][j] - dr[mt][nt][j] + p2;
+        }
       }
   __syncthreads();
   for (int e = threadIdx.x; e < NREG * 64; e += blockDim.x) {
@@ -864,9 +931,12 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
   }
 }
 
-// gW[t][o][i][s][s2] = sum over the blocks of term t, FP64 accumulation in block order
+// gW[t][o][i][s][s2] = sum over the blocks of term t, FP64 accumulation in block
+// order, times the inverse operand scales of the split products (null maxima:
+// unscaled FP32 products)
 template <int MT, int NT>
-__global__ void k_net_wgrad_fin(const float *part, int nblk, int cin, int cout, float2 *gW) {
+__global__ void k_net_wgrad_fin(const float *part, int nblk, int cin, int cout, float2 *gW, const unsigned *gmax,
+                                const unsigned *amax) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 9 * cout * cin * 16
   const int total = 9 * cout * cin * 16;
   if (idx >= total) return;
@@ -900,6 +970,12 @@ __global__ void k_net_wgrad_fin(const float *part, int nblk, int cin, int cout, 
   for (int k = 0; k < kAcc; ++k) {
     sr += re[k];
     si += im[k];
+  }
+  if (gmax) {
+    const double inv = 1.0 / ((double)pow2_scale(__uint_as_float(*gmax), 14) *
+                              (double)pow2_scale(__uint_as_float(*amax), 14));
+    sr *= inv;
+    si *= inv;
   }
   gW[idx] = make_float2((float)sr, (float)si);
 }
@@ -1126,6 +1202,7 @@ struct Net : Precond {
   bool use_split = true;
   int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
   int split_ns = 32;            // sites per wave of the split kernel (QCU_NET_SPLIT_SHAPE)
+  bool split_wgrad = true;      // 8 -> 8 weight gradients on split FP16 (QCU_NET_SPLIT_WGRAD=0: FP32)
   DevBuf spk;                   // packed operands: [direction][layer] x kSplitPackBytes
   DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
   DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
@@ -1266,6 +1343,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
+  if (const char *e = getenv("QCU_NET_SPLIT_WGRAD")) split_wgrad = atoi(e) != 0;
   split_ns = net_split_shape_default();
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
@@ -1513,6 +1591,13 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
     a.partials = (float *)wpart.p;
     a.act_zero = k == 0 ? act0_zero_par : -1;
     a.gout_zero = k == nl - 1 ? gout_zero_par : -1;
+    // a layer that is split in both directions has both field maxima: its
+    // input's from the forward pass, its output gradient's from this one
+    const bool wsplit = split_wgrad && split_layer(k, 0) && split_layer(k, 1) && cin == 8 && cout == 8;
+    if (wsplit) {
+      a.amax = split_io(k, 0).in_max;
+      a.gmax = split_io(k, 1).in_max;
+    }
     const int total = 9 * cout * cin * 16;
     if (use_mfma) {
       const int mt = (cout + 3) / 4, nt = (cin + 3) / 4;
@@ -1521,14 +1606,23 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
       float *pp = (float *)wpart.p;
       float2 *dst = (float2 *)gW.p + woff[k];
       const dim3 fg((unsigned)ndiv(total, 256));
-#define QCU_WG_CASE(MT_, NT_)                                                                     \
-  if (mt == MT_ && nt == NT_) {                                                                   \
-    hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_>), grid, dim3(256), 0, s, a, pp);               \
-    hipLaunchKernelGGL((k_net_wgrad_fin<MT_, NT_>), fg, dim3(256), 0, s, (const float *)pp, kWgBlocks, cin, \
-                       cout, dst);                                                                \
-  }
-      QCU_WG_CASE(1, 1) QCU_WG_CASE(1, 2) QCU_WG_CASE(2, 1) QCU_WG_CASE(2, 2)
-#undef QCU_WG_CASE
+      auto run = [&](auto mtc, auto ntc, auto splitc) {
+        constexpr int MT_ = decltype(mtc)::value, NT_ = decltype(ntc)::value;
+        hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_, decltype(splitc)::value>), grid, dim3(256), 0, s, a, pp);
+        hipLaunchKernelGGL((k_net_wgrad_fin<MT_, NT_>), fg, dim3(256), 0, s, (const float *)pp, kWgBlocks, cin, cout,
+                           dst, a.gmax, a.amax);
+      };
+      using I1 = std::integral_constant<int, 1>;
+      using I2 = std::integral_constant<int, 2>;
+      using F = std::false_type;
+      if (mt == 1 && nt == 1) run(I1{}, I1{}, F{});
+      if (mt == 1 && nt == 2) run(I1{}, I2{}, F{});
+      if (mt == 2 && nt == 1) run(I2{}, I1{}, F{});
+      // the split form exists for the 8 -> 8 shape only
+      if (mt == 2 && nt == 2) {
+        if (wsplit) run(I2{}, I2{}, std::true_type{});
+        else run(I2{}, I2{}, F{});
+      }
       QCU_HIP(hipGetLastError());
     } else {
       const int co = cout >= 2 ? 2 : 1;

@@ -53,7 +53,6 @@
 
 namespace qcu {
 
-typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -77,33 +76,6 @@ struct SplitShape<16> {  // v_mfma_f32_16x16x32_f16
   static QCU_D acc_t mfma(h8v a, h8v b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
   static constexpr QCU_HD int row(int r, int g) { return 4 * g + r; }
 };
-
-// exact power of two that brings a largest magnitude mx below 2^top (1 for a
-// zero or non-finite mx; the exponent is clamped so that the factor is finite)
-QCU_HD float pow2_scale(float mx, int top) {
-  if (!(mx > 0.f) || !(mx <= 3.4e38f)) return 1.f;
-  int e;
-  (void)frexpf(mx, &e);  // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
-  e = e < -110 ? -110 : e;
-  return ldexpf(1.f, top - e);
-}
-
-QCU_D h8v as_h8(uint4 u) { return __builtin_bit_cast(h8v, u); }
-
-// the two FP16 halves of a complex FP32 value: hi = (RN(re), RN(im)) packed,
-// lo = the rounded remainders.  The remainder x - hi is exact in FP32; one
-// v_fma_mix_f32 per component forms it from the packed FP16 operand directly.
-QCU_D void split2(float re, float im, unsigned &hi, unsigned &lo) {
-  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-  typedef float f2v __attribute__((ext_vector_type(2)));
-  const h2v h = __builtin_convertvector((f2v){re, im}, h2v);
-  hi = __builtin_bit_cast(unsigned, h);
-  float rr, ri;
-  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rr) : "v"(hi), "v"(re));
-  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ri) : "v"(hi), "v"(im));
-  const h2v l = __builtin_convertvector((f2v){rr, ri}, h2v);
-  lo = __builtin_bit_cast(unsigned, l);
-}
 
 // Packs one layer's weights W[t][o][i][4x4] (complex, FP32) into the A
 // operands A[t][mt][ks][hl][lane] (8 FP16 each; hl 0 = hi, 1 = lo) of shape

@@ -2,6 +2,8 @@
 // shared by its kernels: channel fields, the neighbour and link of each term,
 // and the colour transport.
 #pragma once
+#include <cmath>
+
 #include "qcu_common.h"
 
 namespace qcu {
@@ -59,6 +61,56 @@ QCU_D TermGeo term_geo(const Geom &g, const Site &s, int i, int p, int t) {
   r.jp = q;
   r.slab = r.fwd ? p * 4 + mu : q * 4 + mu;
   r.li = r.fwd ? i : j;
+  return r;
+}
+
+// The same geometry for a term t that is uniform over the wave, without
+// branches: the direction-dependent quantities are selected, not branched on,
+// so that a loop over sites with a fixed term has one straight-line body.  The
+// term's constants are formed once (TermConst, scalar registers) as sums of
+// products and masks rather than selections among struct fields, which the
+// compiler turns into indexed accesses to a stack copy of the struct.
+struct TermConst {
+  int t, mu, fwd, hop;
+  int len, str;  // extent and half-site stride of direction mu (mu > 0)
+  int aper;      // hops across the t boundary carry -1
+  int sel[3];    // all-ones for the coordinate (y, z, t) along mu, else 0
+};
+QCU_D TermConst term_const(const Geom &g, int t) {
+  TermConst c;
+  c.t = t;
+  c.mu = t >> 1;
+  c.fwd = !(t & 1);
+  c.hop = t < 8;
+  const int m1 = c.mu == 1, m2 = c.mu == 2, m3 = c.mu == 3;
+  c.len = m1 * g.ly + m2 * g.lz + m3 * g.lt;
+  c.str = m1 * g.lxh + m2 * g.lxh * g.ly + m3 * g.lxh * g.ly * g.lz;
+  c.aper = m3 * (g.tbc < 0);
+  c.sel[0] = -m1;
+  c.sel[1] = -m2;
+  c.sel[2] = -m3;
+  return c;
+}
+QCU_D TermGeo term_geo_u(const Geom &g, const Site &s, int i, int p, const TermConst &k) {
+  // x: the even-odd neighbour in the half-site index
+  const int xe = 1 - s.xodd;
+  const int jx = k.fwd ? i + s.xodd - s.xodd * (int)(s.xh == g.lxh - 1) * g.lxh
+                       : i - xe + xe * (int)(s.xh == 0) * g.lxh;
+  // y, z, t: a stride with periodic wrap
+  // the coordinate along mu by masks (a selection among the Site fields
+  // becomes an indexed access to a stack copy of the Site)
+  const int c = (s.y & k.sel[0]) | (s.z & k.sel[1]) | (s.t & k.sel[2]);
+  const int cross = k.fwd ? (int)(c == k.len - 1) : (int)(c == 0);
+  const int jo = k.fwd ? i + k.str - cross * k.len * k.str : i - k.str + cross * k.len * k.str;
+  TermGeo r;
+  r.j = !k.hop ? i : k.mu == 0 ? jx : jo;
+  r.jp = k.hop ? 1 - p : p;
+  r.hop = k.hop;
+  r.fwd = k.fwd;
+  // the site term gets a valid link (slab 0 at its own site), never applied
+  r.slab = !k.hop ? 0 : k.fwd ? p * 4 + k.mu : (1 - p) * 4 + k.mu;
+  r.li = k.fwd ? i : r.j;
+  r.ph = (k.hop && k.aper && cross) ? -1.f : 1.f;
   return r;
 }
 
@@ -147,6 +199,35 @@ int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s);
 bool net_split_can_fuse(int ns);
 constexpr size_t kSplitPackBytes = (size_t)9 * 2 * 4 * 2 * 64 * 16;  // A operands of an 8 -> 8 layer
 constexpr size_t kFusePackBytes = (size_t)3 * 4 * 2 * 64 * 16;       // those of the fused 8 -> 1 mixing
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+// exact power of two that brings a largest magnitude mx below 2^top (1 for a
+// zero or non-finite mx; the exponent is clamped so that the factor is finite)
+QCU_HD float pow2_scale(float mx, int top) {
+  if (!(mx > 0.f) || !(mx <= 3.4e38f)) return 1.f;
+  int e;
+  (void)frexpf(mx, &e);  // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
+  e = e < -110 ? -110 : e;
+  return ldexpf(1.f, top - e);
+}
+
+QCU_D h8v as_h8(uint4 u) { return __builtin_bit_cast(h8v, u); }
+
+// the two FP16 halves of a complex FP32 value: hi = (RN(re), RN(im)) packed,
+// lo = the rounded remainders.  The remainder x - hi is exact in FP32; one
+// v_fma_mix_f32 per component forms it from the packed FP16 operand directly.
+QCU_D void split2(float re, float im, unsigned &hi, unsigned &lo) {
+  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const h2v h = __builtin_convertvector((f2v){re, im}, h2v);
+  hi = __builtin_bit_cast(unsigned, h);
+  float rr, ri;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rr) : "v"(hi), "v"(re));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ri) : "v"(hi), "v"(im));
+  const h2v l = __builtin_convertvector((f2v){rr, ri}, h2v);
+  lo = __builtin_bit_cast(unsigned, l);
+}
 
 // Records the largest |component| a wave wrote into *slot: the wave maximum,
 // then one atomic maximum on the float bits (non-negative floats order like

@@ -304,6 +304,34 @@ def test_split_layers_follow_the_input_scale(orc, scale):
     net.close()
 
 
+# The 8 -> 8 weight gradients run on the FP16 matrix cores with split
+# operands, scaled by powers of two from the recorded field maxima; the
+# gradient is bilinear in (input, output gradient).
+@pytest.mark.parametrize("sx,sg", [(1.0, 1.0), (1e-15, 1e15), (1e8, 1e-12)])
+def test_split_weight_gradient(orc, monkeypatch, sx, sg):
+    """weight gradient with split-FP16 8 -> 8 products: gW(sx x, sg g) / (sx sg)
+    against the oracle's gW(x, g) (1e-4, as the FP32 kernels) and against the
+    FP32 matrix-core weight gradient (1e-5)"""
+    L, chan, tb = (8, 4, 4, 8), (1, 8, 8, 8, 1), -1
+    U, lat, net, W = _setup(orc, L, chan, tb, 210)
+    monkeypatch.setenv("QCU_NET_SPLIT_WGRAD", "0")
+    from qcu_amd.lattice import LearnedPreconditioner
+    ref32 = LearnedPreconditioner(lat, chan, seed=210)
+    ref32.set_weights(W)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 211)
+    g = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 212)
+    _, acts = orc.net_forward(x, W, chan, U, L, tb)
+    _, gW_ref = orc.net_backward(g, W, chan, U, L, acts, tb)
+    gW, _ = net.backward(dev(x * sx), dev(g * sg), want_input_grad=False)
+    gW = gW / (sx * sg)
+    gW32, _ = ref32.backward(dev(x), dev(g), want_input_grad=False)
+    e = (rel(gW, gW_ref), rel(gW, gW32))
+    log(f"split weight gradient, scales {sx:g} {sg:g}: vs oracle {e[0]:.1e}, vs FP32 kernel {e[1]:.1e}")
+    assert e[0] < 1e-4 and e[1] < 1e-5
+    ref32.close()
+    net.close()
+
+
 def test_replacing_the_network_recaptures_graphs(orc):
     """a solve with a new network (after the previous one was destroyed) must not
     replay graphs captured with the old one: graph and eager runs agree bit for bit"""
