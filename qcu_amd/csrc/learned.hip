@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad(NetWgradArgs a) {
 // works on one contiguous stretch of sites (their y neighbours included).
 constexpr int kWgBlocks = 1024;  // blocks per term (a multiple of 8)
 
-template <int MT, int NT, bool SPLIT>
+template <int MT, int NT, bool SPLIT, bool LDS>
 __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *part) {
   const Geom g = a.g;
   const int b = blockIdx.x;
@@ -784,7 +784,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     Geo r;
     int k = grp * 4 + q;
     const bool ok = k < nsite;
-    if (!ok) k = grp * 4;  // a valid site; its contribution is zeroed
+    if (!ok) k = 0;  // a valid site; its contribution is zeroed
     k += site0;
     r.p = k >= g.vh ? 1 : 0;
     r.i = k - r.p * g.vh;
@@ -878,6 +878,97 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
           di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br - bi, di[mt][nt], 0, 0, 0);
         }
   };
+  if constexpr (LDS) {
+    // The block's 4 waves take 4 consecutive groups, 16 consecutive sites, per
+    // step.  Their output gradient and input channels are loaded by the whole
+    // block with whole lines: thread (plane, site) reads float4 slot plane % 6
+    // of channel plane / 6 at one of the 16 sites, so a wave's load touches 4
+    // planes x 256 contiguous bytes.  They are staged in LDS plane by plane
+    // ([plane][site]), and each wave reads its rows and columns from there.
+    // (A wave that read its matrix-core operands straight from memory would
+    // touch 16 planes x 64 bytes per load, 16 MiB apart, all in one
+    // vector-cache set.)  The next step's loads are issued before the
+    // current step's products.
+    __shared__ float4 sG[48 * 16], sV[48 * 16];
+    const int tid = threadIdx.x;
+    const int fs = tid & 15;  // the thread's site in a step; its planes: (tid >> 4) + 16 r
+    // the thread's three planes (separate variables: arrays of them end up
+    // in scratch memory in some instances)
+    auto plane_base = [&](int r, const SpinorPtr &f, int nch) {
+      const int plane = (tid >> 4) + 16 * r, ch = plane / 6, slot = plane - 6 * ch;
+      return (const float4 *)f.p + ((size_t)(ch < nch ? ch : 0) * 12 + slot) * vh;
+    };
+    const float4 *gb0 = plane_base(0, a.gout, a.cout), *gb1 = plane_base(1, a.gout, a.cout),
+                 *gb2 = plane_base(2, a.gout, a.cout);
+    const float4 *vb0 = plane_base(0, a.act, a.cin), *vb1 = plane_base(1, a.act, a.cin),
+                 *vb2 = plane_base(2, a.act, a.cin);
+    float4 pg0, pg1, pg2, pv0, pv1, pv2;
+    auto fetch = [&](int g0) {
+      int k = g0 * 4 + fs;
+      if (k >= nsite) k = 0;
+      k += site0;
+      const int p = k >= g.vh ? 1 : 0, i = k - p * g.vh;
+      const TermGeo tg = term_geo_u(g, decode(g, i, p), i, p, tcon);
+      const unsigned og = (unsigned)p * 6u * (unsigned)g.vh + (unsigned)i;
+      const unsigned ov = (unsigned)tg.jp * 6u * (unsigned)g.vh + (unsigned)tg.j;
+      pg0 = gb0[og];
+      pg1 = gb1[og];
+      pg2 = gb2[og];
+      pv0 = vb0[ov];
+      pv1 = vb1[ov];
+      pv2 = vb2[ov];
+    };
+    auto links = [&](const Geo &ge, float4 (&l)[3]) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
+    };
+    int g0 = sb * 4;  // the block's first group; block-uniform, as every branch below
+    if (g0 < ngroups) {
+      fetch(g0);
+      Geo ge = geo(g0 + wave);
+      Ops x;
+      links(ge, x.l);
+      const int s = 4 * wave + q;
+#pragma unroll 1
+      while (true) {
+        __syncthreads();  // the previous step's LDS reads are done
+        sG[tid] = pg0;
+        sG[256 + tid] = pg1;
+        sG[512 + tid] = pg2;
+        sV[tid] = pv0;
+        sV[256 + tid] = pv1;
+        sV[512 + tid] = pv2;
+        __syncthreads();
+        const bool more = g0 + nw < ngroups;
+        Geo gn;
+        float4 ln[3];
+        if (more) {
+          fetch(g0 + nw);
+          gn = geo(g0 + nw + wave);
+          links(gn, ln);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const int o = grow[mt] != 0.f ? 4 * mt + o_n : 0;
+          x.g[mt][0] = sG[(o * 6 + rslot) * 16 + s];
+          x.g[mt][1] = sG[(o * 6 + rslot + 1) * 16 + s];
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int ch = gcol[nt] != 0.f ? 4 * nt + o_n : 0;
+          x.v[nt][0] = sV[(ch * 6 + rslot) * 16 + s];
+          x.v[nt][1] = sV[(ch * 6 + rslot + 1) * 16 + s];
+        }
+        compute(x, ge);
+        if (!more) break;
+        g0 += nw;
+        ge = gn;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) x.l[k] = ln[k];
+      }
+    }
+  } else {
   // groups grp, grp + nw, ...: the operands of the next group load into the
   // other operand set while this one multiplies (the last group loads a valid
   // dummy: itself), so no load sits behind a branch
@@ -903,6 +994,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       if (!m2) break;
       grp += nw;
     }
+  }
   }
   constexpr int NREG = MT * NT * 2 * 4;
   // block sum in a fixed order: registers (mt, nt, part, j) of every lane
@@ -1203,6 +1295,7 @@ struct Net : Precond {
   int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
   int split_ns = 32;            // sites per wave of the split kernel (QCU_NET_SPLIT_SHAPE)
   bool split_wgrad = true;      // 8 -> 8 weight gradients on split FP16 (QCU_NET_SPLIT_WGRAD=0: FP32)
+  bool wg_lds = true;           // weight-gradient operands staged through LDS (QCU_NET_WGRAD_LDS=0: direct loads)
   DevBuf spk;                   // packed operands: [direction][layer] x kSplitPackBytes
   DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
   DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
@@ -1344,6 +1437,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
   if (const char *e = getenv("QCU_NET_SPLIT_WGRAD")) split_wgrad = atoi(e) != 0;
+  if (const char *e = getenv("QCU_NET_WGRAD_LDS")) wg_lds = atoi(e) != 0;
   split_ns = net_split_shape_default();
   if (const char *e = getenv("QCU_NET_KERNEL")) use_mfma = strcmp(e, "valu") != 0;
   if (const char *e = getenv("QCU_NET_TILE")) tile = atoi(e);  // 0: site order
@@ -1608,7 +1702,10 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
       const dim3 fg((unsigned)ndiv(total, 256));
       auto run = [&](auto mtc, auto ntc, auto splitc) {
         constexpr int MT_ = decltype(mtc)::value, NT_ = decltype(ntc)::value;
-        hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_, decltype(splitc)::value>), grid, dim3(256), 0, s, a, pp);
+        if (wg_lds)
+          hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_, decltype(splitc)::value, true>), grid, dim3(256), 0, s, a, pp);
+        else
+          hipLaunchKernelGGL((k_net_wgrad_mfma<MT_, NT_, decltype(splitc)::value, false>), grid, dim3(256), 0, s, a, pp);
         hipLaunchKernelGGL((k_net_wgrad_fin<MT_, NT_>), fg, dim3(256), 0, s, (const float *)pp, kWgBlocks, cin, cout,
                            dst, a.gmax, a.amax);
       };
