@@ -813,15 +813,15 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     for (int k = 0; k < 3; ++k)
       x.l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
   };
-  auto compute = [&](const Ops &x, const Geo &ge) {
+  // the products of one group: rows x.g, columns x.v transported by u (a
+  // hop term; the site term leaves them as they are), sites weighted by keep
+  auto products = [&](const Ops &x, float keep, const cf (&u)[9]) {
+    const bool hop = tcon.hop;  // uniform
     // rows: output gradient of channels 4 mt + n / 4, spin n % 4
     cf gv[MT][3];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) spin3(x.g[mt], ge.keep * grow[mt], gv[mt]);
+    for (int mt = 0; mt < MT; ++mt) spin3(x.g[mt], keep * grow[mt], gv[mt]);
     // columns: transported inputs of channels 4 nt + n / 4, spin n % 4
-    cf u[9];
-    const bool hop = tcon.hop;  // uniform
-    if (hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
     cf vv[NT][3];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -878,6 +878,11 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
           di[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br - bi, di[mt][nt], 0, 0, 0);
         }
   };
+  auto compute = [&](const Ops &x, const Geo &ge) {
+    cf u[9];
+    if (tcon.hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
+    products(x, ge.keep, u);
+  };
   if constexpr (LDS) {
     // The block's 4 waves take 4 consecutive groups, 16 consecutive sites, per
     // step.  Their output gradient and input channels are loaded by the whole
@@ -890,6 +895,7 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     // vector-cache set.)  The next step's loads are issued before the
     // current step's products.
     __shared__ float4 sG[48 * 16], sV[48 * 16];
+    __shared__ float2 sM[16 * 9];  // the term's colour matrix at each site of the step
     const int tid = threadIdx.x;
     const int fs = tid & 15;  // the thread's site in a step; its planes: (tid >> 4) + 16 r
     // the thread's three planes (separate variables: arrays of them end up
@@ -903,6 +909,11 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     const float4 *vb0 = plane_base(0, a.act, a.cin), *vb1 = plane_base(1, a.act, a.cin),
                  *vb2 = plane_base(2, a.act, a.cin);
     float4 pg0, pg1, pg2, pv0, pv1, pv2;
+    // threads 0..15 (one per site of the step) also load the site's link and
+    // later build the term's colour matrix there, once per site
+    const bool linker = tid < 16;
+    float4 pl0, pl1, pl2;
+    float pph = 1.f;
     auto fetch = [&](int g0) {
       int k = g0 * 4 + fs;
       if (k >= nsite) k = 0;
@@ -917,18 +928,18 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       pv0 = vb0[ov];
       pv1 = vb1[ov];
       pv2 = vb2[ov];
-    };
-    auto links = [&](const Geo &ge, float4 (&l)[3]) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        l[k] = ld_at<float4>(a.U.a, ((size_t)ge.tg.slab * 3 + k) * a.U.vh * 16, (unsigned)ge.tg.li * 16u);
+      if (linker && tcon.hop) {
+        const size_t ps = (size_t)a.U.vh * 16;
+        pl0 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 0) * ps, (unsigned)tg.li * 16u);
+        pl1 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 1) * ps, (unsigned)tg.li * 16u);
+        pl2 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 2) * ps, (unsigned)tg.li * 16u);
+        pph = tg.ph;
+      }
     };
     int g0 = sb * 4;  // the block's first group; block-uniform, as every branch below
     if (g0 < ngroups) {
       fetch(g0);
-      Geo ge = geo(g0 + wave);
       Ops x;
-      links(ge, x.l);
       const int s = 4 * wave + q;
 #pragma unroll 1
       while (true) {
@@ -939,15 +950,17 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
         sV[tid] = pv0;
         sV[256 + tid] = pv1;
         sV[512 + tid] = pv2;
-        __syncthreads();
-        const bool more = g0 + nw < ngroups;
-        Geo gn;
-        float4 ln[3];
-        if (more) {
-          fetch(g0 + nw);
-          gn = geo(g0 + nw + wave);
-          links(gn, ln);
+        if (linker && tcon.hop) {
+          const float4 raw[3] = {pl0, pl1, pl2};
+          cf m[9];
+          term_matrix(raw, 1, tcon.fwd, pph, m);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) sM[tid * 9 + k] = make_float2(m[k].re, m[k].im);
         }
+        __syncthreads();
+        const float keep = (g0 + wave) * 4 + q < nsite ? 1.f : 0.f;
+        const bool more = g0 + nw < ngroups;
+        if (more) fetch(g0 + nw);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           const int o = grow[mt] != 0.f ? 4 * mt + o_n : 0;
@@ -960,12 +973,17 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
           x.v[nt][0] = sV[(ch * 6 + rslot) * 16 + s];
           x.v[nt][1] = sV[(ch * 6 + rslot + 1) * 16 + s];
         }
-        compute(x, ge);
+        cf u[9];
+        if (tcon.hop) {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const float2 e = sM[s * 9 + k];
+            u[k] = mk(e.x, e.y);
+          }
+        }
+        products(x, keep, u);
         if (!more) break;
         g0 += nw;
-        ge = gn;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) x.l[k] = ln[k];
       }
     }
   } else {
