@@ -883,37 +883,46 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     if (tcon.hop) term_matrix(x.l, 1, ge.tg.fwd, ge.tg.ph, u);
     products(x, ge.keep, u);
   };
+  // shared memory: the staged operands of a step (LDS path), then the block
+  // sum of the epilogue (after a barrier)
+  constexpr int NREG = MT * NT * 2 * 4;
+  constexpr int kStage = LDS ? 2 * 48 * 32 + 32 * 9 / 2 : 0;  // float4s: sG, sV, sM
+  constexpr int kSum = 4 * NREG * 64 / 4;
+  __shared__ float4 smem[kStage > kSum ? kStage : kSum];
   if constexpr (LDS) {
-    // The block's 4 waves take 4 consecutive groups, 16 consecutive sites, per
-    // step.  Their output gradient and input channels are loaded by the whole
-    // block with whole lines: thread (plane, site) reads float4 slot plane % 6
-    // of channel plane / 6 at one of the 16 sites, so a wave's load touches 4
-    // planes x 256 contiguous bytes.  They are staged in LDS plane by plane
-    // ([plane][site]), and each wave reads its rows and columns from there.
-    // (A wave that read its matrix-core operands straight from memory would
-    // touch 16 planes x 64 bytes per load, 16 MiB apart, all in one
-    // vector-cache set.)  The next step's loads are issued before the
-    // current step's products.
-    __shared__ float4 sG[48 * 16], sV[48 * 16];
-    __shared__ float2 sM[16 * 9];  // the term's colour matrix at each site of the step
+    // The block's 4 waves take 8 consecutive groups, 32 consecutive sites, per
+    // step: wave w the groups w and w + 4 of the step.  Their output gradient
+    // and input channels are loaded by the whole block with whole lines:
+    // thread (plane, site) reads float4 slot plane % 6 of channel plane / 6 at
+    // one of the 32 sites, so a wave's load touches 2 planes x 512 contiguous
+    // bytes.  They are staged in LDS plane by plane ([plane][site]), and each
+    // wave reads its rows and columns from there.  (A wave that read its
+    // matrix-core operands straight from memory would touch 16 planes x 64
+    // bytes per load, 16 MiB apart, all in one vector-cache set.)  The next
+    // step's loads are issued before the current step's products.
+    float4 *sG = smem, *sV = smem + 48 * 32;
+    float2 *sM = (float2 *)(smem + 2 * 48 * 32);  // the term's colour matrix at each site of the step
     const int tid = threadIdx.x;
-    const int fs = tid & 15;  // the thread's site in a step; its planes: (tid >> 4) + 16 r
-    // the thread's three planes (separate variables: arrays of them end up
-    // in scratch memory in some instances)
+    const int fs = tid & 31;  // the thread's site in a step; its planes: (tid >> 5) + 8 r
+    // the thread's six planes (separate variables: arrays of them end up in
+    // scratch memory in some instances)
     auto plane_base = [&](int r, const SpinorPtr &f, int nch) {
-      const int plane = (tid >> 4) + 16 * r, ch = plane / 6, slot = plane - 6 * ch;
+      const int plane = (tid >> 5) + 8 * r, ch = plane / 6, slot = plane - 6 * ch;
       return (const float4 *)f.p + ((size_t)(ch < nch ? ch : 0) * 12 + slot) * vh;
     };
     const float4 *gb0 = plane_base(0, a.gout, a.cout), *gb1 = plane_base(1, a.gout, a.cout),
-                 *gb2 = plane_base(2, a.gout, a.cout);
+                 *gb2 = plane_base(2, a.gout, a.cout), *gb3 = plane_base(3, a.gout, a.cout),
+                 *gb4 = plane_base(4, a.gout, a.cout), *gb5 = plane_base(5, a.gout, a.cout);
     const float4 *vb0 = plane_base(0, a.act, a.cin), *vb1 = plane_base(1, a.act, a.cin),
-                 *vb2 = plane_base(2, a.act, a.cin);
-    float4 pg0, pg1, pg2, pv0, pv1, pv2;
-    // threads 0..15 (one per site of the step) also load the site's link and
+                 *vb2 = plane_base(2, a.act, a.cin), *vb3 = plane_base(3, a.act, a.cin),
+                 *vb4 = plane_base(4, a.act, a.cin), *vb5 = plane_base(5, a.act, a.cin);
+    float4 pg0, pg1, pg2, pg3, pg4, pg5, pv0, pv1, pv2, pv3, pv4, pv5;
+    // threads 0..31 (one per site of the step) also load the site's link and
     // later build the term's colour matrix there, once per site
-    const bool linker = tid < 16;
+    const bool linker = tid < 32;
     float4 pl0, pl1, pl2;
     float pph = 1.f;
+    const int nw8 = nsb * 8;  // groups between a block's steps
     auto fetch = [&](int g0) {
       int k = g0 * 4 + fs;
       if (k >= nsite) k = 0;
@@ -925,9 +934,15 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       pg0 = gb0[og];
       pg1 = gb1[og];
       pg2 = gb2[og];
+      pg3 = gb3[og];
+      pg4 = gb4[og];
+      pg5 = gb5[og];
       pv0 = vb0[ov];
       pv1 = vb1[ov];
       pv2 = vb2[ov];
+      pv3 = vb3[ov];
+      pv4 = vb4[ov];
+      pv5 = vb5[ov];
       if (linker && tcon.hop) {
         const size_t ps = (size_t)a.U.vh * 16;
         pl0 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 0) * ps, (unsigned)tg.li * 16u);
@@ -936,20 +951,50 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
         pph = tg.ph;
       }
     };
-    int g0 = sb * 4;  // the block's first group; block-uniform, as every branch below
+    // the products of the wave's group h (0, 1) of the step
+    auto group = [&](int g0, int h) {
+      const int s = 16 * h + 4 * wave + q;
+      Ops x;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int o = grow[mt] != 0.f ? 4 * mt + o_n : 0;
+        x.g[mt][0] = sG[(o * 6 + rslot) * 32 + s];
+        x.g[mt][1] = sG[(o * 6 + rslot + 1) * 32 + s];
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int ch = gcol[nt] != 0.f ? 4 * nt + o_n : 0;
+        x.v[nt][0] = sV[(ch * 6 + rslot) * 32 + s];
+        x.v[nt][1] = sV[(ch * 6 + rslot + 1) * 32 + s];
+      }
+      cf u[9];
+      if (tcon.hop) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const float2 e = sM[s * 9 + k];
+          u[k] = mk(e.x, e.y);
+        }
+      }
+      products(x, (g0 + 4 * h + wave) * 4 + q < nsite ? 1.f : 0.f, u);
+    };
+    int g0 = sb * 8;  // the block's first group; block-uniform, as every branch below
     if (g0 < ngroups) {
       fetch(g0);
-      Ops x;
-      const int s = 4 * wave + q;
 #pragma unroll 1
       while (true) {
         __syncthreads();  // the previous step's LDS reads are done
         sG[tid] = pg0;
         sG[256 + tid] = pg1;
         sG[512 + tid] = pg2;
+        sG[768 + tid] = pg3;
+        sG[1024 + tid] = pg4;
+        sG[1280 + tid] = pg5;
         sV[tid] = pv0;
         sV[256 + tid] = pv1;
         sV[512 + tid] = pv2;
+        sV[768 + tid] = pv3;
+        sV[1024 + tid] = pv4;
+        sV[1280 + tid] = pv5;
         if (linker && tcon.hop) {
           const float4 raw[3] = {pl0, pl1, pl2};
           cf m[9];
@@ -958,34 +1003,15 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
           for (int k = 0; k < 9; ++k) sM[tid * 9 + k] = make_float2(m[k].re, m[k].im);
         }
         __syncthreads();
-        const float keep = (g0 + wave) * 4 + q < nsite ? 1.f : 0.f;
-        const bool more = g0 + nw < ngroups;
-        if (more) fetch(g0 + nw);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const int o = grow[mt] != 0.f ? 4 * mt + o_n : 0;
-          x.g[mt][0] = sG[(o * 6 + rslot) * 16 + s];
-          x.g[mt][1] = sG[(o * 6 + rslot + 1) * 16 + s];
-        }
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int ch = gcol[nt] != 0.f ? 4 * nt + o_n : 0;
-          x.v[nt][0] = sV[(ch * 6 + rslot) * 16 + s];
-          x.v[nt][1] = sV[(ch * 6 + rslot + 1) * 16 + s];
-        }
-        cf u[9];
-        if (tcon.hop) {
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const float2 e = sM[s * 9 + k];
-            u[k] = mk(e.x, e.y);
-          }
-        }
-        products(x, keep, u);
+        const bool more = g0 + nw8 < ngroups;
+        if (more) fetch(g0 + nw8);
+        group(g0, 0);
+        group(g0, 1);
         if (!more) break;
-        g0 += nw;
+        g0 += nw8;
       }
     }
+    __syncthreads();  // the block sum below reuses the staging memory
   } else {
   // groups grp, grp + nw, ...: the operands of the next group load into the
   // other operand set while this one multiplies (the last group loads a valid
@@ -1014,9 +1040,8 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     }
   }
   }
-  constexpr int NREG = MT * NT * 2 * 4;
   // block sum in a fixed order: registers (mt, nt, part, j) of every lane
-  __shared__ float sh[4][NREG][64];
+  float(*sh)[NREG][64] = (float(*)[NREG][64])smem;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
