@@ -916,14 +916,14 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
     const float4 *vb0 = plane_base(0, a.act, a.cin), *vb1 = plane_base(1, a.act, a.cin),
                  *vb2 = plane_base(2, a.act, a.cin), *vb3 = plane_base(3, a.act, a.cin),
                  *vb4 = plane_base(4, a.act, a.cin), *vb5 = plane_base(5, a.act, a.cin);
-    float4 pg0, pg1, pg2, pg3, pg4, pg5, pv0, pv1, pv2, pv3, pv4, pv5;
     // threads 0..31 (one per site of the step) also load the site's link and
-    // later build the term's colour matrix there, once per site
+    // build the term's colour matrix there, once per site
     const bool linker = tid < 32;
-    float4 pl0, pl1, pl2;
-    float pph = 1.f;
     const int nw8 = nsb * 8;  // groups between a block's steps
-    auto fetch = [&](int g0) {
+    // the step's loads go straight into LDS (global_load_lds_dwordx4: lane l's
+    // 16 bytes land at the wave's LDS base + 16 l, so a wave's load fills
+    // planes 2 w + 8 r and 2 w + 8 r + 1 of the step); the links into registers
+    auto issue = [&](int g0, float4 (&l)[3], float &ph) {
       int k = g0 * 4 + fs;
       if (k >= nsite) k = 0;
       k += site0;
@@ -931,24 +931,19 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       const TermGeo tg = term_geo_u(g, decode(g, i, p), i, p, tcon);
       const unsigned og = (unsigned)p * 6u * (unsigned)g.vh + (unsigned)i;
       const unsigned ov = (unsigned)tg.jp * 6u * (unsigned)g.vh + (unsigned)tg.j;
-      pg0 = gb0[og];
-      pg1 = gb1[og];
-      pg2 = gb2[og];
-      pg3 = gb3[og];
-      pg4 = gb4[og];
-      pg5 = gb5[og];
-      pv0 = vb0[ov];
-      pv1 = vb1[ov];
-      pv2 = vb2[ov];
-      pv3 = vb3[ov];
-      pv4 = vb4[ov];
-      pv5 = vb5[ov];
+      typedef void __attribute__((address_space(3))) *lds_t;
+      const int wb = 2 * wave * 32;
+#define QCU_WG_LDS(R_)                                                                                   \
+  __builtin_amdgcn_global_load_lds((const void *)(gb##R_ + og), (lds_t)(sG + wb + 8 * (R_) * 32), 16, 0, 0); \
+  __builtin_amdgcn_global_load_lds((const void *)(vb##R_ + ov), (lds_t)(sV + wb + 8 * (R_) * 32), 16, 0, 0);
+      QCU_WG_LDS(0) QCU_WG_LDS(1) QCU_WG_LDS(2) QCU_WG_LDS(3) QCU_WG_LDS(4) QCU_WG_LDS(5)
+#undef QCU_WG_LDS
       if (linker && tcon.hop) {
         const size_t ps = (size_t)a.U.vh * 16;
-        pl0 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 0) * ps, (unsigned)tg.li * 16u);
-        pl1 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 1) * ps, (unsigned)tg.li * 16u);
-        pl2 = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 2) * ps, (unsigned)tg.li * 16u);
-        pph = tg.ph;
+        l[0] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 0) * ps, (unsigned)tg.li * 16u);
+        l[1] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 1) * ps, (unsigned)tg.li * 16u);
+        l[2] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + 2) * ps, (unsigned)tg.li * 16u);
+        ph = tg.ph;
       }
     };
     // the products of the wave's group h (0, 1) of the step
@@ -978,38 +973,22 @@ __global__ void __launch_bounds__(256) k_net_wgrad_mfma(NetWgradArgs a, float *p
       products(x, (g0 + 4 * h + wave) * 4 + q < nsite ? 1.f : 0.f, u);
     };
     int g0 = sb * 8;  // the block's first group; block-uniform, as every branch below
-    if (g0 < ngroups) {
-      fetch(g0);
 #pragma unroll 1
-      while (true) {
-        __syncthreads();  // the previous step's LDS reads are done
-        sG[tid] = pg0;
-        sG[256 + tid] = pg1;
-        sG[512 + tid] = pg2;
-        sG[768 + tid] = pg3;
-        sG[1024 + tid] = pg4;
-        sG[1280 + tid] = pg5;
-        sV[tid] = pv0;
-        sV[256 + tid] = pv1;
-        sV[512 + tid] = pv2;
-        sV[768 + tid] = pv3;
-        sV[1024 + tid] = pv4;
-        sV[1280 + tid] = pv5;
-        if (linker && tcon.hop) {
-          const float4 raw[3] = {pl0, pl1, pl2};
-          cf m[9];
-          term_matrix(raw, 1, tcon.fwd, pph, m);
+    for (; g0 < ngroups; g0 += nw8) {
+      __syncthreads();  // the previous step's LDS reads are done
+      float4 raw[3];
+      float ph = 1.f;
+      issue(g0, raw, ph);
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's loads, those into LDS included, have landed
+      if (linker && tcon.hop) {
+        cf m[9];
+        term_matrix(raw, 1, tcon.fwd, ph, m);
 #pragma unroll
-          for (int k = 0; k < 9; ++k) sM[tid * 9 + k] = make_float2(m[k].re, m[k].im);
-        }
-        __syncthreads();
-        const bool more = g0 + nw8 < ngroups;
-        if (more) fetch(g0 + nw8);
-        group(g0, 0);
-        group(g0, 1);
-        if (!more) break;
-        g0 += nw8;
+        for (int k = 0; k < 9; ++k) sM[tid * 9 + k] = make_float2(m[k].re, m[k].im);
       }
+      __syncthreads();
+      group(g0, 0);
+      group(g0, 1);
     }
     __syncthreads();  // the block sum below reuses the staging memory
   } else {
