@@ -307,12 +307,18 @@ def test_split_layers_follow_the_input_scale(orc, scale):
 # The 8 -> 8 weight gradients run on the FP16 matrix cores with split
 # operands, scaled by powers of two from the recorded field maxima; the
 # gradient is bilinear in (input, output gradient).
-@pytest.mark.parametrize("sx,sg", [(1.0, 1.0), (1e-15, 1e15), (1e8, 1e-12)])
-def test_split_weight_gradient(orc, monkeypatch, sx, sg):
+# Its operands reach the matrix cores through LDS (default) or by direct
+# loads (QCU_NET_WGRAD_LDS=0); (6, 2, 2, 2) has 48 sites (both parities), so
+# the second 32-site step of the LDS path is partial.
+@pytest.mark.parametrize("lds", ["1", "0"])
+@pytest.mark.parametrize("sx,sg,L", [(1.0, 1.0, (8, 4, 4, 8)), (1e-15, 1e15, (8, 4, 4, 8)),
+                                     (1e8, 1e-12, (8, 4, 4, 8)), (1.0, 1.0, (6, 2, 2, 2))])
+def test_split_weight_gradient(orc, monkeypatch, sx, sg, L, lds):
     """weight gradient with split-FP16 8 -> 8 products: gW(sx x, sg g) / (sx sg)
     against the oracle's gW(x, g) (1e-4, as the FP32 kernels) and against the
     FP32 matrix-core weight gradient (1e-5)"""
-    L, chan, tb = (8, 4, 4, 8), (1, 8, 8, 8, 1), -1
+    monkeypatch.setenv("QCU_NET_WGRAD_LDS", lds)
+    chan, tb = (1, 8, 8, 8, 1), -1
     U, lat, net, W = _setup(orc, L, chan, tb, 210)
     monkeypatch.setenv("QCU_NET_SPLIT_WGRAD", "0")
     from qcu_amd.lattice import LearnedPreconditioner
@@ -326,7 +332,8 @@ def test_split_weight_gradient(orc, monkeypatch, sx, sg):
     gW = gW / (sx * sg)
     gW32, _ = ref32.backward(dev(x), dev(g), want_input_grad=False)
     e = (rel(gW, gW_ref), rel(gW, gW32))
-    log(f"split weight gradient, scales {sx:g} {sg:g}: vs oracle {e[0]:.1e}, vs FP32 kernel {e[1]:.1e}")
+    log(f"split weight gradient L={L} lds={lds}, scales {sx:g} {sg:g}: vs oracle {e[0]:.1e}, "
+        f"vs FP32 kernel {e[1]:.1e}")
     assert e[0] < 1e-4 and e[1] < 1e-5
     ref32.close()
     net.close()
