@@ -187,8 +187,11 @@ QCU_D void transport4(const cf (&m)[9], cf (&v)[4][3]) {
 // from their largest component (the two lanes of a site share it) before the
 // FP16 split.  Even output sites produce the 8 hop-term fields (two M tiles),
 // odd ones the site-term field (one tile, rows 0..7).
-QCU_D void fused_epilogue(const NetSplitArgs &a, const f16v (&acc)[3][2], float oscale, int i, int p, bool ok,
-                          int g, int lane, size_t field, size_t plane) {
+// pre() is called once the accumulators are consumed: the caller issues its
+// next loads there.
+template <class Pre>
+QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&acc)[3][2], float oscale, int i,
+                          int p, bool ok, int g, int lane, size_t field, size_t plane, Pre pre) {
   float mx = 0.f;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc)
@@ -201,6 +204,32 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const f16v (&acc)[3][2], float 
   const float f = oscale * sn;                 // accumulator -> scaled B operand (a power of two)
   const float yscale = a.winv2[0] / sn;        // mixed field -> its true value
   const unsigned off = (unsigned)i * 16u;
+  // the B operands of the mixing, shared by its selections: per K step and
+  // colour, the accumulators' rows of that K step, scaled and split
+  h8v bh[4][3], bl[4][3];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int mt = ks >> 1;
+      unsigned uh[4], ul[4];
+#pragma unroll
+      for (int cl = 0; cl < 2; ++cl)
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          const int r = 4 * (2 * (ks & 1) + cl) + 2 * sl;
+          split2(acc[cc][mt][r] * f, acc[cc][mt][r + 1] * f, uh[2 * cl + sl], ul[2 * cl + sl]);
+        }
+      bh[ks][cc] = as_h8(make_uint4(uh[0], uh[1], uh[2], uh[3]));
+      bl[ks][cc] = as_h8(make_uint4(ul[0], ul[1], ul[2], ul[3]));
+    }
+  // the accumulators are consumed: the caller's next loads go out here
+  __builtin_amdgcn_sched_barrier(0);
+  pre();
+  __builtin_amdgcn_sched_barrier(0);
+  // selection sel: one M tile of the mixing (8 packed operands of 64 lanes:
+  // K step ks, hi / lo); the hop-term operands (selections 0 and 1) are in
+  // LDS, the site term's in global memory (the layer's weights leave no room)
   const int nsel = p == 0 ? 2 : 1;
 #pragma unroll 1
   for (int k = 0; k < nsel; ++k) {
@@ -210,24 +239,19 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const f16v (&acc)[3][2], float 
     for (int cc = 0; cc < 3; ++cc) dy[cc] = (f16v){};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      const int ab = ((sel * 4 + ks) * 2) * 64 + lane;
-      const h8v ah = as_h8(a.A2[ab]), al = as_h8(a.A2[ab + 64]);
-      const int mt = ks >> 1;
+      h8v ah, al;
+      if (p == 0) {
+        ah = as_h8(lA2[((k * 4 + ks) * 2) * 64 + lane]);
+        al = as_h8(lA2[((k * 4 + ks) * 2 + 1) * 64 + lane]);
+      } else {
+        ah = as_h8(a.A2[((8 + ks) * 2) * 64 + lane]);
+        al = as_h8(a.A2[((8 + ks) * 2 + 1) * 64 + lane]);
+      }
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        unsigned uh[4], ul[4];
-#pragma unroll
-        for (int cl = 0; cl < 2; ++cl)
-#pragma unroll
-          for (int sl = 0; sl < 2; ++sl) {
-            const int r = 4 * (2 * (ks & 1) + cl) + 2 * sl;
-            split2(acc[cc][mt][r] * f, acc[cc][mt][r + 1] * f, uh[2 * cl + sl], ul[2 * cl + sl]);
-          }
-        const h8v bh = as_h8(make_uint4(uh[0], uh[1], uh[2], uh[3]));
-        const h8v bl = as_h8(make_uint4(ul[0], ul[1], ul[2], ul[3]));
-        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, dy[cc], 0, 0, 0);
-        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, dy[cc], 0, 0, 0);
-        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, dy[cc], 0, 0, 0);
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[ks][cc], dy[cc], 0, 0, 0);
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[ks][cc], dy[cc], 0, 0, 0);
+        dy[cc] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[ks][cc], dy[cc], 0, 0, 0);
       }
     }
     if (!ok) continue;
@@ -260,9 +284,14 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   static_assert(!FUSE || (NS == 32 && MT == 2), "the fused epilogue is written for the 32-site shape, 8 outputs");
   constexpr int G = 64 / NS, KS = 8 / G, NR = SH::NR;
   constexpr int NA = 9 * MT * KS * 2 * 64;
-  __shared__ uint4 lw[NA];
+  // fused: the next layer's hop-term operands follow the weights (LDS total
+  // 144 + 16 KiB = the CU's 160 KiB)
+  constexpr int NA2 = FUSE ? 2 * 4 * 2 * 64 : 0;
+  __shared__ uint4 lw[NA + NA2];
   if (a.st && a.st->done) return;  // uniform over the grid
   for (int k = threadIdx.x; k < NA; k += blockDim.x) lw[k] = a.A[k];
+  if constexpr (FUSE)
+    for (int k = threadIdx.x; k < NA2; k += blockDim.x) lw[NA + k] = a.A2[k];
   __syncthreads();
   const Geom g = a.g;
   const float sin_ = pow2_scale(__uint_as_float(*a.in_max), 14);
@@ -305,11 +334,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
   const unsigned chan_off = (unsigned)(2 * field);  // one channel (both parities)
   float omax = 0.f;
-#pragma unroll 1
-  for (int c = c0; c < cend; c += cstep) {
-    const int p = a.parity0 + (a.npar == 2 ? (c & 1) : 0);
+  // chunk c: output parity, first site of the lane (lanes past the last site
+  // compute a copy of a valid site and store nothing), diagonal rotation
+  struct Chunk {
+    int p, i, phi;
+    bool ok;
+  };
+  auto chunk_at = [&](int c) {
+    Chunk k;
+    k.p = a.parity0 + (a.npar == 2 ? (c & 1) : 0);
     int cs = a.npar == 2 ? c >> 1 : c;  // chunk in site order
-    int phi = 0;
+    k.phi = 0;
     if (diag) {
       int r1 = cs;  // y fastest, then the plane within the pair, t, the pair
       const int y = r1 % g.ly;
@@ -319,85 +354,131 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
       const int tt = r1 % g.lt, pair = r1 / g.lt;
       const int z = (int)(blockIdx.x & 7) * (g.lz >> 3) + 2 * pair + zz;
       cs = (tt * g.lz + z) * g.ly + y;
-      phi = (y + 2 * z + 3 * tt) % 7;
+      k.phi = (y + 2 * z + 3 * tt) % 7;
     } else if (slab) {
       const int dz = g.lz >> 3, within = cs % cpp, q = cs / cpp;
       cs = ((q / dz) * g.lz + (int)(blockIdx.x & 7) * dz + q % dz) * cpp + within;
     }
     const int base = cs * NS;
-    int i = base + n;
-    const bool ok = i < g.vh;
-    if (!ok) i = base;  // lanes past the last site compute a copy of a valid site and store nothing
-    const Site site = decode(g, i, p);
-    typename SH::acc_t acc[3][MT];
+    k.i = base + n;
+    k.ok = k.i < g.vh;
+    if (!k.ok) k.i = base;
+    return k;
+  };
+  typename SH::acc_t acc[3][MT];
+  // the lane's channel operand in K step ks: input channel G ks + gq at the
+  // term's neighbour; the base of channel G ks is uniform over the wave and
+  // the lane group adds gq channels (a 32-bit offset, checked at launch)
+  auto load_v = [&](float4 (&q)[6], const TermGeo &tg, int ks) {
+    const char *cb = a.in + ((size_t)(2 * G * ks) + tg.jp) * field;
+    const unsigned off = (unsigned)tg.j * 16u + (unsigned)gq * chan_off;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(cb, k * plane, off);
+  };
+  auto load_link = [&](const TermGeo &tg, float4 (&l)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
+  };
+  // one K step: the channel operand in q is unpacked and transported, then q
+  // is refilled with the next step's operand (next(q)) while the products run
+  auto step = [&](float4 (&q)[6], const cf (&m)[9], bool hop, int t, int ks, auto next) {
+    cf v[4][3];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int ci = 3 * s2 + cc;
+        const float4 f = q[ci >> 1];
+        v[s2][cc] = (ci & 1) ? mk(f.z, f.w) : mk(f.x, f.y);
+      }
+    if (hop) {
+      transport4(m, v);  // m carries the input scale
+    } else {
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) v[s2][cc] = sin_ * v[s2][cc];
+    }
+    // the refill is issued here, a whole step ahead of its use: the
+    // scheduling barriers keep the compiler from sinking it below the
+    // products (where it would reuse the operand registers and leave the
+    // next step waiting on it)
+    __builtin_amdgcn_sched_barrier(0);
+    next(q);
+    __builtin_amdgcn_sched_barrier(0);
+    // the two FP16 halves of each colour's B operand
+    unsigned uh[3][4], ul[3][4];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) split2(v[s2][cc].re, v[s2][cc].im, uh[cc][s2], ul[cc][s2]);
+    // M tile by M tile: its two A halves, then three products per colour
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int ab = (((t * MT + mt) * KS + ks) * 2) * 64 + lane;
+      const h8v ah = as_h8(lw[ab]), al = as_h8(lw[ab + 64]);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const h8v bh = as_h8(make_uint4(uh[cc][0], uh[cc][1], uh[cc][2], uh[cc][3]));
+        const h8v bl = as_h8(make_uint4(ul[cc][0], ul[cc][1], ul[cc][2], ul[cc][3]));
+        acc[cc][mt] = SH::mfma(ah, bh, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(ah, bl, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(al, bh, acc[cc][mt]);
+      }
+    }
+  };
+  // the chunk's output: its 8 (or 4) channels, or with FUSE the next layer's
+  // mixed term fields
+  // pre() issues the wave's next loads: before the stores of a plain layer,
+  // once the accumulators are split in the fused epilogue
+  auto finish = [&](const Chunk &k, auto pre) {
+    if constexpr (FUSE) {
+      fused_epilogue(a, lw + NA, acc, oscale, k.i, k.p, k.ok, gq, lane, field, plane, pre);
+    } else {
+      pre();
+      if (!k.ok) return;
+      // lane (n, gq) holds, per M tile and group of 4 registers j, output
+      // channel (NS / 8) mt + row(4 j, gq) / 8, spins sb and sb + 1 with
+      // sb = row(4 j, gq) % 8 / 2, all colours: float4 planes 3 sb / 2 .. + 2
+      const unsigned off = (unsigned)k.i * 16u;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int j = 0; j < NR / 4; ++j) {
+          const int r0 = SH::row(4 * j, gq);
+          const int o = (NS / 8) * mt + r0 / 8, sb = (r0 % 8) / 2;
+          char *ob = a.out + ((size_t)o * 2 + k.p) * field;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            // complex li = 2q, 2q + 1 of the lane's six: spin sb + li / 3, colour li % 3
+            const int l0 = 2 * q, l1 = 2 * q + 1;
+            const int e0 = 4 * j + 2 * (l0 / 3), e1 = 4 * j + 2 * (l1 / 3);
+            float4 f;
+            f.x = acc[l0 % 3][mt][e0] * oscale;
+            f.y = acc[l0 % 3][mt][e0 + 1] * oscale;
+            f.z = acc[l1 % 3][mt][e1] * oscale;
+            f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
+            omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+            st_at<float4>(ob, (size_t)(3 * (sb / 2) + q) * plane, off, f);
+          }
+        }
+    }
+  };
+  auto clear = [&]() {
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) acc[cc][mt] = (typename SH::acc_t){};
-    // the lane's channel operand in K step ks: input channel G ks + gq at the
-    // term's neighbour; the base of channel G ks is uniform over the wave and
-    // the lane group adds gq channels (a 32-bit offset, checked at launch)
-    auto load_v = [&](float4 (&q)[6], const TermGeo &tg, int ks) {
-      const char *cb = a.in + ((size_t)(2 * G * ks) + tg.jp) * field;
-      const unsigned off = (unsigned)tg.j * 16u + (unsigned)gq * chan_off;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(cb, k * plane, off);
-    };
-    auto load_link = [&](const TermGeo &tg, float4 (&l)[3]) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
-    };
-    // one K step: the channel operand in q is unpacked and transported, then q
-    // is refilled with the next step's operand (next(q)) while the products run
-    auto step = [&](float4 (&q)[6], const cf (&m)[9], bool hop, int t, int ks, auto next) {
-      cf v[4][3];
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2)
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          const int ci = 3 * s2 + cc;
-          const float4 f = q[ci >> 1];
-          v[s2][cc] = (ci & 1) ? mk(f.z, f.w) : mk(f.x, f.y);
-        }
-      if (hop) {
-        transport4(m, v);  // m carries the input scale
-      } else {
-#pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2)
-#pragma unroll
-          for (int cc = 0; cc < 3; ++cc) v[s2][cc] = sin_ * v[s2][cc];
-      }
-      // the refill is issued here, a whole step ahead of its use: the
-      // scheduling barriers keep the compiler from sinking it below the
-      // products (where it would reuse the operand registers and leave the
-      // next step waiting on it)
-      __builtin_amdgcn_sched_barrier(0);
-      next(q);
-      __builtin_amdgcn_sched_barrier(0);
-      // the two FP16 halves of each colour's B operand
-      unsigned uh[3][4], ul[3][4];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2) split2(v[s2][cc].re, v[s2][cc].im, uh[cc][s2], ul[cc][s2]);
-      // M tile by M tile: its two A halves, then three products per colour
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const int ab = (((t * MT + mt) * KS + ks) * 2) * 64 + lane;
-        const h8v ah = as_h8(lw[ab]), al = as_h8(lw[ab + 64]);
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          const h8v bh = as_h8(make_uint4(uh[cc][0], uh[cc][1], uh[cc][2], uh[cc][3]));
-          const h8v bl = as_h8(make_uint4(ul[cc][0], ul[cc][1], ul[cc][2], ul[cc][3]));
-          acc[cc][mt] = SH::mfma(ah, bh, acc[cc][mt]);
-          acc[cc][mt] = SH::mfma(ah, bl, acc[cc][mt]);
-          acc[cc][mt] = SH::mfma(al, bh, acc[cc][mt]);
-        }
-      }
-    };
-    if constexpr (DIAG) {
-      // (diag holds: the launcher picks this instance only when it does)
+  };
+  if constexpr (DIAG) {
+    // (diag holds: the launcher picks this instance only when it does)
+#pragma unroll 1
+    for (int c = c0; c < cend; c += cstep) {
+      const Chunk ck = chunk_at(c);
+      const int i = ck.i, p = ck.p;
+      const Site site = decode(g, i, p);
+      clear();
       // Diagonal schedule.  Slot sigma holds: 0 the local terms (x+, site,
       // x-), then y- 1, z- 2, t- 3, t+ 4, z+ 5, y+ 6, i.e. +-a_mu for the
       // backward / forward hop with a = (y 1, z 2, t 3).  At step k a wave
@@ -410,7 +491,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
       int nseq = 0;
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
-        const int sg = (k + phi) % 7;
+        const int sg = (k + ck.phi) % 7;
         seq |= slot_terms[sg] << (4 * nseq);
         nseq += slot_len[sg];
       }
@@ -438,63 +519,50 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
           tg = ng;
         }
       }
-    } else {
+      finish(ck, []() {});
+    }
+  } else {
     // term outer, K step inner: each term's link is loaded and its third row
-    // rebuilt once, the next term's link while this one runs
-    TermGeo tg = term_geo(g, site, i, p, 0);
-    float4 lraw[3];
-    load_link(tg, lraw);
-    float4 vb[6];
-    load_v(vb, tg, 0);
+    // rebuilt once, the next term's link while this one runs.  The first
+    // loads of a wave's next chunk are issued before the current chunk's
+    // output is formed and stored, so that they overlap it.
+    float4 lraw[3], vb[6];
+    Chunk ck{};
+    Site site{};
+    TermGeo tg{};
+    auto begin = [&](int c) {
+      ck = chunk_at(c);
+      site = decode(g, ck.i, ck.p);
+      tg = term_geo(g, site, ck.i, ck.p, 0);
+      load_link(tg, lraw);
+      load_v(vb, tg, 0);
+    };
+    if (c0 < cend) begin(c0);
 #pragma unroll 1
-    for (int t = 0; t < 9; ++t) {
-      cf m[9];
-      term_matrix(lraw, tg.hop, tg.fwd, tg.ph * sin_, m);
-      TermGeo ng = tg;
-      if (t + 1 < 9) {
-        ng = term_geo(g, site, i, p, t + 1);
-        if (ng.hop) load_link(ng, lraw);
-      }
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        step(vb, m, tg.hop, t, ks, [&](float4(&q)[6]) {
-          if (ks + 1 < KS) load_v(q, tg, ks + 1);
-          else if (t + 1 < 9) load_v(q, ng, 0);
-        });
-      }
-      tg = ng;
-    }
-    }
-    if constexpr (FUSE) {
-      fused_epilogue(a, acc, oscale, i, p, ok, gq, lane, field, plane);
-      continue;
-    }
-    if (ok) {
-      // lane (n, gq) holds, per M tile and group of 4 registers j, output
-      // channel (NS / 8) mt + row(4 j, gq) / 8, spins sb and sb + 1 with
-      // sb = row(4 j, gq) % 8 / 2, all colours: float4 planes 3 sb / 2 .. + 2
-      const unsigned off = (unsigned)i * 16u;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int j = 0; j < NR / 4; ++j) {
-          const int r0 = SH::row(4 * j, gq);
-          const int o = (NS / 8) * mt + r0 / 8, sb = (r0 % 8) / 2;
-          char *ob = a.out + ((size_t)o * 2 + p) * field;
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            // complex li = 2q, 2q + 1 of the lane's six: spin sb + li / 3, colour li % 3
-            const int l0 = 2 * q, l1 = 2 * q + 1;
-            const int e0 = 4 * j + 2 * (l0 / 3), e1 = 4 * j + 2 * (l1 / 3);
-            float4 f;
-            f.x = acc[l0 % 3][mt][e0] * oscale;
-            f.y = acc[l0 % 3][mt][e0 + 1] * oscale;
-            f.z = acc[l1 % 3][mt][e1] * oscale;
-            f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
-            omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
-            st_at<float4>(ob, (size_t)(3 * (sb / 2) + q) * plane, off, f);
-          }
+    for (int c = c0; c < cend; c += cstep) {
+      const Chunk cur = ck;
+      clear();
+#pragma unroll 1
+      for (int t = 0; t < 9; ++t) {
+        cf m[9];
+        term_matrix(lraw, tg.hop, tg.fwd, tg.ph * sin_, m);
+        TermGeo ng = tg;
+        if (t + 1 < 9) {
+          ng = term_geo(g, site, cur.i, cur.p, t + 1);
+          if (ng.hop) load_link(ng, lraw);
         }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          step(vb, m, tg.hop, t, ks, [&](float4(&q)[6]) {
+            if (ks + 1 < KS) load_v(q, tg, ks + 1);
+            else if (t + 1 < 9) load_v(q, ng, 0);
+          });
+        }
+        tg = ng;
+      }
+      finish(cur, [&]() {
+        if (c + cstep < cend) begin(c + cstep);
+      });
     }
   }
   if (a.out_max) record_max(a.out_max, omax);

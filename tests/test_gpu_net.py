@@ -228,6 +228,40 @@ def test_split_fp16_layers_match_oracle_and_fp32_kernel(orc, monkeypatch, L, cha
     net.close()
 
 
+# A wave walks several chunks when the lattice has more 32-site chunks than
+# the grid has waves (256 CUs x 8): 16x16x18x32 has 2 x 36864 / 32 = 4608, so
+# waves take 2 or 3 chunks, and each chunk's first loads are issued while the
+# previous chunk's output is stored.  Forward (plain epilogue) and apply_odd
+# (the fused epilogue) against the oracle and the FP32 matrix-core kernel.
+@pytest.mark.parametrize("order", [0, 1])
+def test_split_layers_several_chunks_per_wave(orc, monkeypatch, order):
+    """split-FP16 layers on a lattice with 2-3 chunks per wave"""
+    L, chan, tb = (16, 16, 18, 32), (1, 8, 8, 8, 1), -1
+    monkeypatch.setenv("QCU_NET_SPLIT_ORDER", str(order))
+    U, lat, net, W = _setup(orc, L, chan, tb, 210)
+    monkeypatch.setenv("QCU_NET_SPLIT", "0")
+    from qcu_amd.lattice import LearnedPreconditioner
+    ref32 = LearnedPreconditioner(lat, chan, seed=210)
+    ref32.set_weights(W)
+    x = orc.gauss((1, 2, orc.vol(L) // 2, 4, 3), 211)
+    ref, _ = orc.net_forward(x, W, chan, U, L, tb)
+    out = host(net.forward(dev(x))).ravel()
+    out32 = host(ref32.forward(dev(x))).ravel()
+    xo = np.ascontiguousarray(x[0, 1])
+    full = np.zeros_like(x)
+    full[0, 1] = xo
+    ref_odd, _ = orc.net_forward(full, W, chan, U, L, tb)
+    got_odd = host(net.apply_odd(dev(xo))).ravel()
+    got_odd32 = host(ref32.apply_odd(dev(xo))).ravel()
+    e = (rel(out, ref), rel(out, out32), rel(got_odd, ref_odd.reshape(2, -1)[1]), rel(got_odd, got_odd32))
+    log(f"split L={L} order {order}: forward {e[0]:.1e} (FP32 kernel {e[1]:.1e}), "
+        f"apply_odd {e[2]:.1e} ({e[3]:.1e})")
+    assert e[0] < 2e-5 and e[2] < 2e-5
+    assert e[1] < 2e-6 and e[3] < 2e-6
+    ref32.close()
+    net.close()
+
+
 # Both MFMA shapes of the split kernel (32 sites per wave on 32x32x16, 16 on
 # 16x16x32) and its chunk orders: 2 walks a z slab per XCD; 3 (16 sites, Lx
 # 32, Lz a multiple of 16) gives each wave one x row and rotates its term
