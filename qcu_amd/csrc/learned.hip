@@ -1136,22 +1136,39 @@ __global__ void k_net_transpose(const float2 *W, float2 *Wt, int cin, int cout) 
 }
 
 // embed an odd-site vector (precision P) as channel 0 of a full FP32 field
+// (rec: receives the largest |component|, or null; then 4 sites per thread
+// and one atomic per block of 256 threads)
 template <int P>
-__global__ void k_net_embed(SpinorPtr full, SpinorPtr odd, int vh, const SysState *st) {
+__global__ void __launch_bounds__(256) k_net_embed(SpinorPtr full, SpinorPtr odd, int vh, const SysState *st,
+                                                  unsigned *rec = nullptr) {
   if (st && st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
   typedef typename real_of<P>::type T;
-  cplx<T> v[12];
-  SpinorIO<P>::load(odd, i, v);
-  cf w[12], z[12];
+  const int per = rec ? 4 : 1;
+  float mx = 0.f;
+  for (int k = 0; k < per; ++k) {
+    const int i = (blockIdx.x * per + k) * blockDim.x + threadIdx.x;
+    if (i >= vh) break;
+    cplx<T> v[12];
+    SpinorIO<P>::load(odd, i, v);
+    cf w[12], z[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) {
-    w[k] = mk((float)v[k].re, (float)v[k].im);
-    z[k] = mk(0.f, 0.f);
+    for (int c = 0; c < 12; ++c) {
+      w[c] = mk((float)v[c].re, (float)v[c].im);
+      z[c] = mk(0.f, 0.f);
+      mx = fmaxf(mx, fmaxf(fabsf(w[c].re), fabsf(w[c].im)));
+    }
+    SpinorIO<P32>::store(chan_field(full, 0, 1, vh), i, w);
+    SpinorIO<P32>::store(chan_field(full, 0, 0, vh), i, z);
   }
-  SpinorIO<P32>::store(chan_field(full, 0, 1, vh), i, w);
-  SpinorIO<P32>::store(chan_field(full, 0, 0, vh), i, z);
+  if (!rec) return;  // uniform
+  __shared__ float red[256];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(rec, __float_as_uint(red[0]));
 }
 
 template <int P>
@@ -1342,6 +1359,14 @@ struct Net : Precond {
   }
   int pack_split(int dir, hipStream_t s);
   int reset_max(int dir, hipStream_t s);
+  // the first layer (1 -> 8) on an input that is zero on the even sites runs
+  // on the split kernel net_layer_first when its input's largest component
+  // was recorded (load_input)
+  bool first_split() const { return use_mfma && use_split && nl >= 2 && chan[0] == 1 && chan[1] == 8; }
+  bool input_fresh = false;  // load_input ran since the last forward pass: maxima cleared, input's recorded
+  // act[0] <- the odd vector in (precision prec) as channel 0, zero on the
+  // even sites; clears the field maxima of the next forward pass first
+  int load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s);
   // the last layer (8 -> 1 channels) can be folded into the epilogue of the
   // 8 -> 8 layer before it when only the network output is needed
   bool fusable() const {
@@ -1617,6 +1642,10 @@ int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hip
 // Packs the split operands of every split layer in direction dir (0: the
 // weights, 1: their transposes in Wt) and records the weight version packed.
 int Net::pack_split(int dir, hipStream_t s) {
+  if (dir == 0 && first_split()) {
+    const SplitIO io = split_io(0, 0);
+    QCU_TRY(net_pack_first((const float2 *)W.p + woff[0], (void *)io.A, (float *)io.winv, s));
+  }
   for (int k = 0; k < nl; ++k) {
     if (!split_layer(k, dir)) continue;
     const SplitIO io = split_io(k, dir);
@@ -1644,12 +1673,15 @@ int Net::reset_max(int dir, hipStream_t s) {
 }
 
 int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts) {
-  bool any = false;
+  bool any = first_split();
   for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
+  // the first layer on the split kernel needs the input's maximum, recorded by load_input
+  const bool first = first_split() && zero_par0 == 0 && input_fresh;
   if (any) {
     if (packed_ver != wver) QCU_TRY(pack_split(0, s));
-    QCU_TRY(reset_max(0, s));
+    if (!input_fresh) QCU_TRY(reset_max(0, s));
   }
+  input_fresh = false;
   // fused: the 8 -> 8 layer before the last writes the last layer's mixed
   // term fields, so act[nl - 1] is never written and the last layer is only
   // its gather pass
@@ -1663,12 +1695,26 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
     }
     // the kernel before a split layer records the largest component it writes
     SplitIO io = split_io(k, 0);
-    if (!split_layer(k, 0)) io.A = nullptr;
+    if (!split_layer(k, 0) && !(k == 0 && first)) io.A = nullptr;
     if (!(k + 1 < nl && split_layer(k + 1, 0))) io.out_max = nullptr;
     if (fuse && k == nl - 2) {
       io.A2 = fpk.p;
       io.winv2 = (const float *)swinv.p + 2 * kNetMaxLayers;
       io.mix = mixb.p;
+    }
+    if (k == 0 && first) {
+      NetFirstArgs f;
+      f.in = (const char *)act_ptr(0).p;
+      f.out = (char *)act_ptr(1).p;
+      f.A = (const uint4 *)io.A;
+      f.winv = io.winv;
+      f.in_max = io.in_max;
+      f.out_max = io.out_max;
+      f.U = lat->links(P32);
+      f.g = lat->g;
+      f.st = st;
+      QCU_TRY(net_layer_first(f, s));
+      continue;
     }
     QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s,
                   k == 0 ? zero_par0 : -1, any ? &io : nullptr));
@@ -1775,16 +1821,29 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
   return 0;
 }
 
-int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
+int Net::load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s) {
   const int vh = lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
   const SpinorPtr a0 = act_ptr(0);
+  bool any = first_split();
+  for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
+  if (any) QCU_TRY(reset_max(0, s));
+  unsigned *rec = first_split() ? (unsigned *)split_io(0, 0).in_max : nullptr;
+  const unsigned nbr = rec ? (unsigned)ndiv(vh, 4 * bs) : nb;
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_net_embed<P64>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
-    case P32: hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
-    default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nb), dim3(bs), 0, S.stream, a0, in, vh, st); break;
+    case P64: hipLaunchKernelGGL(k_net_embed<P64>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
+    case P32: hipLaunchKernelGGL(k_net_embed<P32>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
+    default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
   }
   QCU_HIP(hipGetLastError());
+  input_fresh = any;
+  return 0;
+}
+
+int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
+  const int vh = lat->g.vh, bs = 256;
+  const unsigned nb = (unsigned)ndiv(vh, bs);
+  QCU_TRY(load_input(in, prec, st, S.stream));
   QCU_TRY(forward(st, S.stream, true, 0, false));
   const SpinorPtr an = act_ptr(nl);
   switch (prec) {
@@ -1851,8 +1910,7 @@ struct NetTrainer {
       return 0;
     }
     // y = K b / |b|
-    hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), odd(0), vh, nullptr);
-    QCU_HIP(hipGetLastError());
+    QCU_TRY(net.load_input(odd(0), P32, nullptr, s));
     // only the odd part of the output enters the loss: the last layer runs on
     // the odd sites only, as in the preconditioner
     QCU_TRY(net.forward(nullptr, s, true, 0));
@@ -2103,8 +2161,7 @@ int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
   NetTrainer tr{net, s};
   hipLaunchKernelGGL(k_load_odd, dim3(nb), dim3(bs), 0, s, tr.odd(0), (const double2 *)in, vh);
   QCU_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
-  QCU_HIP(hipGetLastError());
+  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
   QCU_TRY(net.forward(nullptr, s, true, 0, false));
   SpinorPtr o1 = net.act_ptr(net.nl);
   o1.p = (char *)o1.p + (size_t)vh * 96;  // channel 0, parity 1
@@ -2195,8 +2252,7 @@ int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms) {
   const unsigned nb = (unsigned)ndiv(vh, bs);
   NetTrainer tr{net, s};
   auto once = [&]() {
-    hipLaunchKernelGGL(k_net_embed<P32>, dim3(nb), dim3(bs), 0, s, net.act_ptr(0), tr.odd(0), vh, nullptr);
-    QCU_HIP(hipGetLastError());
+    QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
     QCU_TRY(net.forward(nullptr, s, true, 0, false));
     hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, tr.odd(1), net.act_ptr(net.nl), vh, nullptr);
     QCU_HIP(hipGetLastError());

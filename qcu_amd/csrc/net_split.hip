@@ -568,6 +568,238 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   if (a.out_max) record_max(a.out_max, omax);
 }
 
+// ---------------------------------------------------------------------------
+// The first layer (1 -> 8 channels) on an input that is zero on the even
+// sites (the preconditioner's and training's input), split FP16 on
+// v_mfma_f32_32x32x16_f16.  Even output sites take the 8 hop terms, odd ones
+// the site term.  Per colour the even sites are the GEMM
+//   D[(o, s, re/im)][site] = sum over (term t, spin s2, re/im) of W_t[o][s][s2] V_t[s2][site]
+// with K step ks holding the forward (lane group 0) and backward (group 1)
+// hop in direction ks: each lane transports the one input channel from its
+// term's neighbour.  The odd sites are one more K step, the site term on
+// lane group 0 (group 1 zero).  A wave owns 32 sites and produces both
+// parities of their 8 output channels.
+// Packs W[t][o][0][4x4] as A[ks][mt][hl][lane] (ks 0..3 the hop pairs, 4
+// the site term; 8 FP16 each), scaled by a power of two; winv[0] receives
+// its inverse.  One block.
+__global__ void __launch_bounds__(256) k_net_pack_first(const float2 *W, uint4 *Apk, float *winv) {
+  __shared__ float red[256];
+  const int tid = threadIdx.x;
+  const int nw = 9 * 8 * 16;
+  float mx = 0.f;
+  for (int k = tid; k < nw; k += 256) mx = fmaxf(mx, fmaxf(fabsf(W[k].x), fabsf(W[k].y)));
+  red[tid] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmaxf(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  const float sw = pow2_scale(red[0], 14);
+  if (tid == 0) winv[0] = 1.f / sw;
+  for (int e = tid; e < kFirstPackOps; e += 256) {
+    const int lane = e & 63;
+    int r = e >> 6;
+    const int hl = r & 1;
+    r >>= 1;
+    const int mt = r & 1, ks = r >> 1;
+    const int m = lane & 31, g = lane >> 5;
+    const int o = 4 * mt + (m >> 3), s = (m >> 1) & 3, ri = m & 1;
+    const int t = ks < 4 ? 2 * ks + g : (g == 0 ? 8 : -1);
+    h8v out;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int s2 = q >> 1, rin = q & 1;
+      float v = 0.f;
+      if (t >= 0) {
+        const float2 w = W[((size_t)t * 8 + o) * 16 + s * 4 + s2];
+        v = ri == 0 ? (rin == 0 ? w.x : -w.y) : (rin == 0 ? w.y : w.x);
+      }
+      v *= sw;
+      const _Float16 h = (_Float16)v;
+      out[q] = hl == 0 ? h : (_Float16)(v - (float)h);
+    }
+    Apk[e] = __builtin_bit_cast(uint4, out);
+  }
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
+  typedef SplitShape<32> SH;
+  __shared__ uint4 lw[kFirstPackOps];
+  if (a.st && a.st->done) return;  // uniform over the grid
+  for (int k = threadIdx.x; k < kFirstPackOps; k += blockDim.x) lw[k] = a.A[k];
+  __syncthreads();
+  const Geom g = a.g;
+  const float sin_ = pow2_scale(__uint_as_float(*a.in_max), 14);
+  const float oscale = a.winv[0] / sin_;
+  const int lane = threadIdx.x & 63, gq = lane >> 5, n = lane & 31;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nchunk = (g.vh + 31) / 32;
+  const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
+  const char *in1 = a.in + field;  // the input's odd sites
+  float omax = 0.f;
+  f16v acc[3][2];
+  auto clear = [&]() {
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) acc[cc][0] = acc[cc][1] = (f16v){};
+  };
+  // products of one K step: the lane's transported (and scaled) 4 spins x 3
+  // colours, split, against the two M tiles of the step
+  auto products = [&](const cf (&v)[4][3], int ks) {
+    unsigned uh[3][4], ul[3][4];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) split2(v[s2][cc].re, v[s2][cc].im, uh[cc][s2], ul[cc][s2]);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const int ab = ((ks * 2 + mt) * 2) * 64 + lane;
+      const h8v ah = as_h8(lw[ab]), al = as_h8(lw[ab + 64]);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const h8v bh = as_h8(make_uint4(uh[cc][0], uh[cc][1], uh[cc][2], uh[cc][3]));
+        const h8v bl = as_h8(make_uint4(ul[cc][0], ul[cc][1], ul[cc][2], ul[cc][3]));
+        acc[cc][mt] = SH::mfma(ah, bh, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(ah, bl, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(al, bh, acc[cc][mt]);
+      }
+    }
+  };
+  auto unpack = [](const float4 (&q)[6], cf (&v)[4][3]) {
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int ci = 3 * s2 + cc;
+        const float4 f = q[ci >> 1];
+        v[s2][cc] = (ci & 1) ? mk(f.z, f.w) : mk(f.x, f.y);
+      }
+  };
+  // lane (n, gq) holds spins 2 gq, 2 gq + 1 of the 8 output channels of its
+  // site: float4 planes 3 gq .. 3 gq + 2 of each channel
+  auto store = [&](int i, int p) {
+    const unsigned off = (unsigned)i * 16u;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int o = 4 * mt + j;
+        char *ob = a.out + ((size_t)o * 2 + p) * field;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int l0 = 2 * q, l1 = 2 * q + 1;
+          const int e0 = 4 * j + 2 * (l0 / 3), e1 = 4 * j + 2 * (l1 / 3);
+          float4 f;
+          f.x = acc[l0 % 3][mt][e0] * oscale;
+          f.y = acc[l0 % 3][mt][e0 + 1] * oscale;
+          f.z = acc[l1 % 3][mt][e1] * oscale;
+          f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
+          omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+          st_at<float4>(ob, (size_t)(3 * gq + q) * plane, off, f);
+        }
+      }
+  };
+  auto load_v = [&](float4 (&q)[6], int j) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(in1, k * plane, (unsigned)j * 16u);
+  };
+  auto load_link = [&](const TermGeo &tg, float4 (&l)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
+  };
+  // chunk c: the lane's site (lanes past the last site compute a copy of a
+  // valid site and store nothing) and the first hop pair's loads
+  int i = 0;
+  bool ok = false;
+  Site se{};
+  TermGeo tg{};
+  float4 lraw[3], vb[6];
+  auto begin = [&](int c) {
+    const int base = c * 32;
+    i = base + n;
+    ok = i < g.vh;
+    if (!ok) i = base;
+    se = decode(g, i, 0);
+    tg = term_geo_u(g, se, i, 0, term_const(g, gq));
+    load_link(tg, lraw);
+    load_v(vb, tg.j);
+  };
+  const int cstep = gridDim.x * WAVES;
+  int c = blockIdx.x * WAVES + wave;
+  if (c < nchunk) begin(c);
+#pragma unroll 1
+  for (; c < nchunk; c += cstep) {
+    // even output sites: K step ks = the hops in direction ks, forward on
+    // lane group 0, backward on group 1; the next step's link and
+    // neighbour are loaded while this step's products run
+    clear();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      cf m[9], v[4][3];
+      term_matrix(lraw, 1, tg.fwd, tg.ph * sin_, m);
+      unpack(vb, v);
+      transport4(m, v);
+      __builtin_amdgcn_sched_barrier(0);
+      if (ks < 3) {
+        tg = term_geo_u(g, se, i, 0, term_const(g, 2 * (ks + 1) + gq));
+        load_link(tg, lraw);
+        load_v(vb, tg.j);
+      } else {
+        load_v(vb, i);  // the odd pass: the site's own input
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      products(v, ks);
+    }
+    if (ok) store(i, 0);
+    // odd output sites: the site term on lane group 0; the next chunk's first
+    // loads go out first
+    clear();
+    {
+      cf v[4][3];
+      unpack(vb, v);
+      const int io = i;
+      const bool oko = ok;
+      __builtin_amdgcn_sched_barrier(0);
+      if (c + cstep < nchunk) begin(c + cstep);
+      __builtin_amdgcn_sched_barrier(0);
+      const float f = gq == 0 ? sin_ : 0.f;
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) v[s2][cc] = f * v[s2][cc];
+      products(v, 4);
+      if (oko) store(io, 1);
+    }
+  }
+  if (a.out_max) record_max(a.out_max, omax);
+}
+
+int net_pack_first(const float2 *W, void *Apk, float *winv, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_pack_first, dim3(1), dim3(256), 0, s, W, (uint4 *)Apk, winv);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int net_layer_first(const NetFirstArgs &a, hipStream_t s) {
+  QCU_CHECK((size_t)a.g.vh * 96 < ((size_t)1 << 32), QCU_ERR_DIMS, "net: lattice too large for the first layer");
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    QCU_HIP(hipGetDevice(&dev));
+    QCU_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu <= 0) ncu = 256;
+  }
+  // 8 waves per block, one block per CU (213 registers: two waves per SIMD)
+  constexpr int waves = 8;
+  const int nchunk = (a.g.vh + 31) / 32;
+  int nblk = ncu;
+  if (nchunk < nblk * waves) nblk = (nchunk + waves - 1) / waves;
+  hipLaunchKernelGGL((k_net_first<waves>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
 // MFMA shape of the split kernel: 32 sites per wave (v_mfma_f32_32x32x16_f16)
 // or 16 (v_mfma_f32_16x16x32_f16); QCU_NET_SPLIT_SHAPE=16 selects the second
 // (read when a network is created: its packed weights follow the shape)

@@ -199,6 +199,23 @@ int net_pack_fuse(const float2 *W, void *Apk, float *winv, hipStream_t s);
 bool net_split_can_fuse(int ns);
 constexpr size_t kSplitPackBytes = (size_t)9 * 2 * 4 * 2 * 64 * 16;  // A operands of an 8 -> 8 layer
 constexpr size_t kFusePackBytes = (size_t)3 * 4 * 2 * 64 * 16;       // those of the fused 8 -> 1 mixing
+constexpr int kFirstPackOps = 5 * 2 * 2 * 64;                         // A operands of the first (1 -> 8) layer
+
+// the first layer (1 -> 8 channels) on an input that is zero on the even
+// sites: act[0] -> act[1], split FP16 (net_layer_first)
+struct NetFirstArgs {
+  const char *in;           // the input channel field (both parities; only the odd sites are read)
+  char *out;                // the 8 output channel fields: (c * 2 + parity) * vh * 96 bytes
+  const uint4 *A;           // packed operands (net_pack_first)
+  const float *winv;        // inverse of the weights' power-of-two scale
+  const unsigned *in_max;   // largest |component| of the input (float bits)
+  unsigned *out_max;        // receives that of the output (or null)
+  LinkPtr U;                // FP32 links
+  Geom g;
+  const SysState *st;       // early-out flag of a captured solver iteration (or null)
+};
+int net_pack_first(const float2 *W, void *Apk, float *winv, hipStream_t s);
+int net_layer_first(const NetFirstArgs &a, hipStream_t s);
 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
