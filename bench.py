@@ -38,6 +38,7 @@ XPAY_BYTES = {0: 192, 1: 96, 2: 52}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp16_clover_32x32x32x64_traffic.json")
 NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_split_32x32x32x64_traffic.json")
 HOP64_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_clover_32x32x32x64_traffic.json")
+HOP64W_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_wilson_32x32x32x64_traffic.json")
 
 
 def traffic_of(path):
@@ -218,6 +219,7 @@ def main():
     achieved = alg / (ms_hop * 1e-3) / 1e9
     traffic = traffic_of(PROFILE_TRAFFIC)
     traffic64 = traffic_of(HOP64_TRAFFIC)
+    traffic64w = traffic_of(HOP64W_TRAFFIC)
     ms64 = lat.bench_hop(prec=FP64, clover=True, xpay=True, iters=50)
     alg64 = (HOP_BYTES[0] + CLOVER_BYTES[0] + XPAY_BYTES[0]) * vh
     ms64_plain = lat.bench_hop(prec=FP64, clover=False, xpay=False, iters=50)
@@ -292,11 +294,17 @@ def main():
                          "ms_per_launch": round(ms_hop, 5), "alg_bytes_per_launch": alg},
         "hop_fp64_wilson": {"ms_per_launch": round(ms64_plain, 5),
                             "achieved_GBps": round(HOP_BYTES[0] * vh / (ms64_plain * 1e-3) / 1e9, 1),
-                            "GFLOPs": round(1320 * vh / (ms64_plain * 1e-3) / 1e9, 1)},
+                            "GFLOPs": round(1320 * vh / (ms64_plain * 1e-3) / 1e9, 1),
+                            "traffic": traffic64w,
+                            "hbm_GBps": None if traffic64w is None else round(traffic64w / (ms64_plain * 1e-3) / 1e9, 1),
+                            "hbm_frac": None if traffic64w is None else
+                            round(traffic64w / (ms64_plain * 1e-3) / 1e9 / PEAK_GBS, 3)},
         "hop_fp64_clover_xpay": {"ms_per_launch": round(ms64, 5),
                                  "achieved_GBps": round(alg64 / (ms64 * 1e-3) / 1e9, 1),
                                  "traffic": traffic64,
-                                 "hbm_GBps": None if traffic64 is None else round(traffic64 / (ms64 * 1e-3) / 1e9, 1)},
+                                 "hbm_GBps": None if traffic64 is None else round(traffic64 / (ms64 * 1e-3) / 1e9, 1),
+                                 "hbm_frac": None if traffic64 is None else
+                                 round(traffic64 / (ms64 * 1e-3) / 1e9 / PEAK_GBS, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -566,14 +566,17 @@ __global__ void __launch_bounds__(256) k_net_mix(NetMixArgs a) {
 
 struct NetGatherArgs {
   SpinorPtr mix;  // the 9 term fields of k_net_mix
-  SpinorPtr out;  // the output channel
+  SpinorPtr out;  // the output channel (FP32 channel layout), or with DIRECT an odd vector of precision P
   LinkPtr U;      // FP32 links
   Geom g;
   int parity0;    // first output parity (the grid's y index adds to it)
   const SysState *st;
 };
 
-// one thread per output site: the site term plus the 8 transported hop terms
+// one thread per output site: the site term plus the 8 transported hop terms.
+// DIRECT: the odd output goes straight into a vector of precision P (the
+// solver's), instead of channel 0 of the FP32 output block.
+template <int P = P32, bool DIRECT = false>
 __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
@@ -594,7 +597,15 @@ __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = acc[k] + v[k];
   }
-  SpinorIO<P32>::store(chan_field(a.out, 0, p, g.vh), i, acc);
+  if constexpr (DIRECT) {
+    typedef typename real_of<P>::type T;
+    cplx<T> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = mk<T>((T)acc[k].re, (T)acc[k].im);
+    SpinorIO<P>::store(a.out, i, v);
+  } else {
+    SpinorIO<P32>::store(chan_field(a.out, 0, p, g.vh), i, acc);
+  }
 }
 
 // weight gradient partials: for term t, input channel ci, outputs og..og+CO-1,
@@ -1374,7 +1385,8 @@ struct Net : Precond {
            net_split_can_fuse(split_ns) && mixb.p != nullptr;
   }
   // the second pass of the single-output layer: transports and sums the term fields in mixb
-  int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s);
+  // direct_prec >= 0: out is an odd vector of that precision (the odd output parity only)
+  int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec = -1);
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
   SpinorPtr act_ptr(int layer) const {
@@ -1390,7 +1402,10 @@ struct Net : Precond {
             const SysState *st, hipStream_t s, int zero_par = -1, const SplitIO *io = nullptr);
   // zero_par0: parity on which the network input is zero (-1: none);
   // keep_acts false: only act[nl] is needed afterwards (the last layer may be fused)
-  int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts = true);
+  // direct (with odd_only_last, keep_acts false): the odd output may go straight into this
+  // vector of precision direct_prec; *wrote tells whether it did (fused last layer)
+  int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts = true,
+              const SpinorPtr *direct = nullptr, int direct_prec = P32, bool *wrote = nullptr);
   // gout_zero_par: parity on which gout_last is zero; act0_zero_par: parity on
   // which the network input act[0] is zero (-1: none)
   int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1,
@@ -1624,7 +1639,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
 }
 
 // act[0] holds the input; fills act[1..nl] (act[nl] = output)
-int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s) {
+int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec) {
   NetGatherArgs ga;
   ga.mix.p = mixb.p;
   ga.mix.norm = nullptr;
@@ -1634,7 +1649,14 @@ int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hip
   ga.g = lat->g;
   ga.parity0 = parity0;
   ga.st = st;
-  hipLaunchKernelGGL(k_net_gather, dim3((unsigned)ndiv(lat->g.vh, 256), nparity), dim3(256), 0, s, ga);
+  const dim3 grid((unsigned)ndiv(lat->g.vh, 256), nparity);
+  QCU_CHECK(direct_prec < 0 || (parity0 == 1 && nparity == 1), QCU_ERR_ARG, "net: direct output is the odd parity");
+  switch (direct_prec) {
+    case P64: hipLaunchKernelGGL((k_net_gather<P64, true>), grid, dim3(256), 0, s, ga); break;
+    case P32: hipLaunchKernelGGL((k_net_gather<P32, true>), grid, dim3(256), 0, s, ga); break;
+    case P16: hipLaunchKernelGGL((k_net_gather<P16, true>), grid, dim3(256), 0, s, ga); break;
+    default: hipLaunchKernelGGL((k_net_gather<P32, false>), grid, dim3(256), 0, s, ga); break;
+  }
   QCU_HIP(hipGetLastError());
   return 0;
 }
@@ -1672,7 +1694,9 @@ int Net::reset_max(int dir, hipStream_t s) {
   return 0;
 }
 
-int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts) {
+int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts,
+                 const SpinorPtr *direct, int direct_prec, bool *wrote) {
+  if (wrote) *wrote = false;
   bool any = first_split();
   for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
   // the first layer on the split kernel needs the input's maximum, recorded by load_input
@@ -1690,7 +1714,12 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
     const bool last = k == nl - 1;
     const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
     if (fuse && last) {
-      QCU_TRY(gather(act_ptr(k + 1), p0, np, st, s));
+      if (direct) {
+        QCU_TRY(gather(*direct, p0, np, st, s, direct_prec));
+        if (wrote) *wrote = true;
+      } else {
+        QCU_TRY(gather(act_ptr(k + 1), p0, np, st, s));
+      }
       break;
     }
     // the kernel before a split layer records the largest component it writes
@@ -1844,7 +1873,9 @@ int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
   const int vh = lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
   QCU_TRY(load_input(in, prec, st, S.stream));
-  QCU_TRY(forward(st, S.stream, true, 0, false));
+  bool wrote = false;
+  QCU_TRY(forward(st, S.stream, true, 0, false, &out, prec, &wrote));
+  if (wrote) return 0;  // the fused last layer wrote out
   const SpinorPtr an = act_ptr(nl);
   switch (prec) {
     case P64: hipLaunchKernelGGL(k_net_extract<P64>, dim3(nb), dim3(bs), 0, S.stream, out, an, vh, st); break;
@@ -2251,11 +2282,16 @@ int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms) {
   const int vh = net.lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
   NetTrainer tr{net, s};
+  // as in the solver: the fused last layer writes the odd output vector itself
   auto once = [&]() {
     QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
-    QCU_TRY(net.forward(nullptr, s, true, 0, false));
-    hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, tr.odd(1), net.act_ptr(net.nl), vh, nullptr);
-    QCU_HIP(hipGetLastError());
+    bool wrote = false;
+    const SpinorPtr o1 = tr.odd(1);
+    QCU_TRY(net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote));
+    if (!wrote) {
+      hipLaunchKernelGGL(k_net_extract<P32>, dim3(nb), dim3(bs), 0, s, tr.odd(1), net.act_ptr(net.nl), vh, nullptr);
+      QCU_HIP(hipGetLastError());
+    }
     return 0;
   };
   // a deterministic input of unit size (all-zero operands would run the
