@@ -340,9 +340,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
     int p, i, phi;
     bool ok;
   };
+  // The two parities of a site chunk are chunks c and c ^ 1.  Which is which
+  // alternates every 4 chunks, so that the two waves of a SIMD (waves w and
+  // w + 4 of a block) take different parities: the fused epilogue does twice
+  // the work for even output sites.
   auto chunk_at = [&](int c) {
     Chunk k;
-    k.p = a.parity0 + (a.npar == 2 ? (c & 1) : 0);
+    k.p = a.parity0 + (a.npar == 2 ? ((c ^ (c >> 2)) & 1) : 0);
     int cs = a.npar == 2 ? c >> 1 : c;  // chunk in site order
     k.phi = 0;
     if (diag) {
