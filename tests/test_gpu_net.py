@@ -131,10 +131,14 @@ def test_training_gradient_vs_oracle(orc, L, chan, csw):
     assert e_l < 1e-4 and e_g < 1e-3
 
 
-def test_trained_preconditioner_reduces_iterations(orc):
+# The preconditioned inner solve in each precision: the last layer's gather
+# writes the network output straight into the solver's vector of that precision.
+@pytest.mark.parametrize("prec_name", ["fp32", "fp64", "fp16"])
+def test_trained_preconditioner_reduces_iterations(orc, prec_name):
     """Adam on the GPU lowers the loss; the preconditioned solve needs fewer
     iterations than the plain one and still meets |b - M x|/|b| <= 1e-10"""
-    from qcu_amd.lattice import FP32, Lattice, LearnedPreconditioner
+    from qcu_amd.lattice import PREC, Lattice, LearnedPreconditioner
+    prec = PREC[prec_name]
     L, kappa, csw, tb = (8, 8, 8, 16), 0.1245, 1.0, -1
     U = orc.gauge_random(L, 0.25, 130)
     lat = Lattice(L, kappa, csw, tb)
@@ -148,11 +152,11 @@ def test_trained_preconditioner_reduces_iterations(orc):
     log(f"loss {l0:.4f} -> {l1:.4f} in {secs:.1f} s; per epoch {np.round(losses, 4).tolist()}")
     assert l1 < 0.8 * l0
     b = dev(orc.gauss((2, vh, 4, 3), 140))
-    x0, r0 = lat.solve(b, tol=1e-10, prec=FP32)
-    x1, r1 = lat.solve(b, tol=1e-10, prec=FP32, precond=net)
+    x0, r0 = lat.solve(b, tol=1e-10, prec=prec)
+    x1, r1 = lat.solve(b, tol=1e-10, prec=prec, precond=net)
     t0 = orc.true_residual(host(x0), host(b), U, L, kappa, csw, tb)
     t1 = orc.true_residual(host(x1), host(b), U, L, kappa, csw, tb)
-    log(f"plain {r0.iters} iters ({r0.seconds * 1e3:.1f} ms, true res {t0:.1e}); "
+    log(f"{prec_name}: plain {r0.iters} iters ({r0.seconds * 1e3:.1f} ms, true res {t0:.1e}); "
         f"learned {r1.iters} iters ({r1.seconds * 1e3:.1f} ms, true res {t1:.1e}, fallback {r1.fallback})")
     assert r0.converged and r1.converged and t0 <= 1e-10 and t1 <= 1e-10
     assert not r1.fallback
