@@ -59,9 +59,12 @@ def test_net_forward_and_backward_vs_oracle(orc, L, chan, tb):
     assert e_gw < 1e-4
 
 
+# apply_odd takes the split first layer when the first layer is 1 -> 8
+# (followed here by no split layer, by an 8 -> 4 split layer, or by nothing
+# but the last layer), the FP32 kernels for 1 -> 4.
+@pytest.mark.parametrize("chan", [(1, 4, 4, 1), (1, 8, 1), (1, 8, 4, 1)])
 @pytest.mark.parametrize("L", [(8, 8, 4, 4), (32, 4, 4, 4), (6, 2, 2, 2)])
-def test_net_apply_odd_is_odd_part_of_forward(orc, L):
-    chan = (1, 4, 4, 1)
+def test_net_apply_odd_is_odd_part_of_forward(orc, L, chan):
     U, lat, net, W = _setup(orc, L, chan, -1, 110)
     xo = orc.gauss((orc.vol(L) // 2, 4, 3), 111)
     full = np.zeros((1, 2) + xo.shape, complex)
@@ -69,7 +72,7 @@ def test_net_apply_odd_is_odd_part_of_forward(orc, L):
     ref, _ = orc.net_forward(full, W, chan, U, L, -1)
     got = host(net.apply_odd(dev(xo)))
     e = rel(got.ravel(), ref.reshape(2, -1)[1])
-    log(f"apply_odd vs oracle {e:.1e}")
+    log(f"apply_odd chan={chan} L={L} vs oracle {e:.1e}")
     assert e < 2e-5
 
 
