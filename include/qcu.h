@@ -120,7 +120,7 @@ typedef struct QcuSolveParam_s {
   int max_iter;          /* BiCGSTAB iterations (sloppy iterations count) */
   int precision;         /* QCU_PREC_*: inner (sloppy) precision */
   double reliable_delta; /* reliable update when |r| < delta |r_last_update| */
-  int chunk;             /* iterations between host checks (graph length) */
+  int chunk;             /* iterations between host checks (graph length); 0: 8, or 4 with precond */
   int use_graph;         /* capture the chunk as a hipGraph and replay it */
   void *precond;         /* QcuNetHandle or NULL */
   int legacy_stop;       /* 1: stop on Re<r,r> < tol absolute (reference rule) */

@@ -155,7 +155,7 @@ class Lattice:
         _lib.check(self._lib.qcuTrueResidual(self.h, px, pb, C.byref(r)))
         return r.value
 
-    def solve(self, b, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
+    def solve(self, b, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=0,
               use_graph=True, precond=None, legacy_stop=False, x=None, method="bicgstab"):
         """x with |b - M x| / |b| <= tol; method "bicgstab" (any precision,
         optional preconditioner) or "cgnr" (CG on the normal equations, FP64)"""
@@ -230,7 +230,7 @@ class Batch:
         except Exception:
             pass
 
-    def solve(self, bs, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=8,
+    def solve(self, bs, tol=1e-10, max_iter=20000, prec=FP64, reliable_delta=0.1, chunk=0,
               use_graph=True, xs=None, method="bicgstab"):
         if len(bs) != self.nsys or (xs is not None and len(xs) != self.nsys):
             raise ValueError(f"expected {self.nsys} right-hand sides (and solutions)")
