@@ -1350,6 +1350,7 @@ struct Net : Precond {
   DevBuf smax;                  // [direction][layer + 1] largest |component| of each field (float bits)
   DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
   DevBuf fpk;                   // packed operands of the fused single-output layer
+  DevBuf p2pk;                  // combined operands of the second layer (net_pack_second), then 2 inverse scales
   unsigned long long wver = 1;  // weight version (bumped on every change of W)
   unsigned long long packed_ver = 0;
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
@@ -1374,6 +1375,10 @@ struct Net : Precond {
   // on the split kernel net_layer_first when its input's largest component
   // was recorded (load_input)
   bool first_split() const { return use_mfma && use_split && nl >= 2 && chan[0] == 1 && chan[1] == 8; }
+  // ... and then the second layer (8 -> 8) on net_layer_second, the first
+  // layer's odd sites left implicit, when only the network output is needed
+  bool second_split() const { return first_split() && nl >= 3 && chan[2] == 8 && split_layer(1, 0) && split_ns == 32; }
+  float *p2winv() const { return (float *)((char *)p2pk.p + (size_t)kFirstPackOps * 16); }
   bool input_fresh = false;  // load_input ran since the last forward pass: maxima cleared, input's recorded
   // act[0] <- the odd vector in (precision prec) as channel 0, zero on the
   // even sites; clears the field maxima of the next forward pass first
@@ -1495,6 +1500,7 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_TRY(smax.alloc((size_t)2 * (kNetMaxLayers + 1) * 4));
   QCU_TRY(swinv.alloc((size_t)(2 * kNetMaxLayers + 1) * 4));
   QCU_TRY(fpk.alloc(kFusePackBytes));
+  QCU_TRY(p2pk.alloc((size_t)kFirstPackOps * 16 + 16));
   QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
@@ -1668,6 +1674,8 @@ int Net::pack_split(int dir, hipStream_t s) {
     const SplitIO io = split_io(0, 0);
     QCU_TRY(net_pack_first((const float2 *)W.p + woff[0], (void *)io.A, (float *)io.winv, s));
   }
+  if (dir == 0 && second_split())
+    QCU_TRY(net_pack_second((const float2 *)W.p + woff[0], (const float2 *)W.p + woff[1], p2pk.p, p2winv(), s));
   for (int k = 0; k < nl; ++k) {
     if (!split_layer(k, dir)) continue;
     const SplitIO io = split_io(k, dir);
@@ -1710,6 +1718,8 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
   // term fields, so act[nl - 1] is never written and the last layer is only
   // its gather pass
   const bool fuse = !keep_acts && odd_only_last && fusable();
+  // the first two layers as net_layer_first (even sites) + net_layer_second
+  const bool second = first && !keep_acts && second_split() && !(fuse && nl - 2 == 1);
   for (int k = 0; k < nl; ++k) {
     const bool last = k == nl - 1;
     const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
@@ -1742,7 +1752,25 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
       f.U = lat->links(P32);
       f.g = lat->g;
       f.st = st;
-      QCU_TRY(net_layer_first(f, s));
+      QCU_TRY(net_layer_first(f, s, second));
+      continue;
+    }
+    if (k == 1 && second) {
+      NetSecondArgs q;
+      q.in = (const char *)act_ptr(0).p;
+      q.l1 = (const char *)act_ptr(1).p;
+      q.out = (char *)act_ptr(2).p;
+      q.A = (const uint4 *)p2pk.p;
+      q.winv = p2winv();
+      q.Ah = (const uint4 *)io.A;
+      q.winvh = io.winv;
+      q.in_max = split_io(0, 0).in_max;
+      q.l1_max = io.in_max;
+      q.out_max = io.out_max;
+      q.U = lat->links(P32);
+      q.g = lat->g;
+      q.st = st;
+      QCU_TRY(net_layer_second(q, s));
       continue;
     }
     QCU_TRY(layer(act_ptr(k), act_ptr(k + 1), (const float2 *)W.p + woff[k], chan[k], chan[k + 1], p0, np, st, s,

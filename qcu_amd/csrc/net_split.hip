@@ -626,7 +626,9 @@ __global__ void __launch_bounds__(256) k_net_pack_first(const float2 *W, uint4 *
   }
 }
 
-template <int WAVES>
+// EVEN_ONLY: the even output sites only (the odd ones are then never read:
+// the next layer takes their site term from the input itself, k_net_second)
+template <int WAVES, bool EVEN_ONLY>
 __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
   typedef SplitShape<32> SH;
   __shared__ uint4 lw[kFirstPackOps];
@@ -737,6 +739,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
     // even output sites: K step ks = the hops in direction ks, forward on
     // lane group 0, backward on group 1; the next step's link and
     // neighbour are loaded while this step's products run
+    const int ie = i;
+    const bool oke = ok;
     clear();
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
@@ -749,13 +753,16 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
         tg = term_geo_u(g, se, i, 0, term_const(g, 2 * (ks + 1) + gq));
         load_link(tg, lraw);
         load_v(vb, tg.j);
+      } else if constexpr (EVEN_ONLY) {
+        if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads
       } else {
         load_v(vb, i);  // the odd pass: the site's own input
       }
       __builtin_amdgcn_sched_barrier(0);
       products(v, ks);
     }
-    if (ok) store(i, 0);
+    if (oke) store(ie, 0);
+    if constexpr (EVEN_ONLY) continue;
     // odd output sites: the site term on lane group 0; the next chunk's first
     // loads go out first
     clear();
@@ -785,7 +792,7 @@ int net_pack_first(const float2 *W, void *Apk, float *winv, hipStream_t s) {
   return 0;
 }
 
-int net_layer_first(const NetFirstArgs &a, hipStream_t s) {
+int net_layer_first(const NetFirstArgs &a, hipStream_t s, bool even_only) {
   QCU_CHECK((size_t)a.g.vh * 96 < ((size_t)1 << 32), QCU_ERR_DIMS, "net: lattice too large for the first layer");
   static int ncu = 0;
   if (!ncu) {
@@ -799,7 +806,322 @@ int net_layer_first(const NetFirstArgs &a, hipStream_t s) {
   const int nchunk = (a.g.vh + 31) / 32;
   int nblk = ncu;
   if (nchunk < nblk * waves) nblk = (nchunk + waves - 1) / waves;
-  hipLaunchKernelGGL((k_net_first<waves>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  if (even_only) hipLaunchKernelGGL((k_net_first<waves, true>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  else hipLaunchKernelGGL((k_net_first<waves, false>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The second layer (8 -> 8) fused with what the first one leaves implicit.
+// With the network input zero on the even sites, the first layer's output is
+// its site term W1_8 in(y) at odd y, and its 8 hop terms at even y.  So, by
+// linearity (the spin matrices commute with the colour transports), the
+// second layer's output is
+//   even x: sum_t (W2_t W1_8 + W2_8 W1_t) T_t in(x + t)      (t = the 8 hops)
+//   odd x:  sum_t W2_t T_t L1(x + t) + W2_8 W1_8 in(x)        (L1 at even sites)
+// The even sites read only the one-channel input (like k_net_first), and the
+// first layer writes only its even sites.  Products are split FP16 as in
+// k_net_layer_split; the combined matrices E_t = W2_t W1_8 + W2_8 W1_t and
+// S = W2_8 W1_8 are formed here (FP64 sums, rounded to FP32) and packed like
+// k_net_first's operands: A[ks][mt][hl][lane], ks 0..3 the hop pairs of E, 4
+// the site term S; winv[0], winv[1] the inverse scales of E and S.
+__global__ void __launch_bounds__(256) k_net_pack_second(const float2 *W1, const float2 *W2, uint4 *Apk,
+                                                         float *winv) {
+  __shared__ float2 cm[9 * 8 * 16];  // E_0..7, then S: [t][o][s][s2]
+  __shared__ float red[2][256];
+  const int tid = threadIdx.x;
+  float mxe = 0.f, mxs = 0.f;
+  for (int e = tid; e < 9 * 8 * 16; e += 256) {
+    const int t = e >> 7, o = (e >> 4) & 7, s = (e >> 2) & 3, s2 = e & 3;
+    double re = 0.0, im = 0.0;
+    // sum_i sum_s' A[o][i][s][s'] B[i][s'][s2], A of the second layer (cin 8),
+    // B of the first (cin 1); W[t][o][i][s][s2] = W[((t cout + o) cin + i) 16 + 4 s + s2]
+    auto acc = [&](int ta, int tb) {
+      for (int i = 0; i < 8; ++i)
+        for (int sp = 0; sp < 4; ++sp) {
+          const float2 a = W2[(((size_t)ta * 8 + o) * 8 + i) * 16 + s * 4 + sp];
+          const float2 b = W1[((size_t)tb * 8 + i) * 16 + sp * 4 + s2];
+          re += (double)a.x * b.x - (double)a.y * b.y;
+          im += (double)a.x * b.y + (double)a.y * b.x;
+        }
+    };
+    if (t < 8) {
+      acc(t, 8);
+      acc(8, t);
+    } else {
+      acc(8, 8);
+    }
+    const float2 v = make_float2((float)re, (float)im);
+    cm[e] = v;
+    const float m = fmaxf(fabsf(v.x), fabsf(v.y));
+    if (t < 8) mxe = fmaxf(mxe, m);
+    else mxs = fmaxf(mxs, m);
+  }
+  red[0][tid] = mxe;
+  red[1][tid] = mxs;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      red[0][tid] = fmaxf(red[0][tid], red[0][tid + h]);
+      red[1][tid] = fmaxf(red[1][tid], red[1][tid + h]);
+    }
+    __syncthreads();
+  }
+  const float swe = pow2_scale(red[0][0], 14), sws = pow2_scale(red[1][0], 14);
+  if (tid == 0) {
+    winv[0] = 1.f / swe;
+    winv[1] = 1.f / sws;
+  }
+  for (int e = tid; e < kFirstPackOps; e += 256) {
+    const int lane = e & 63;
+    int r = e >> 6;
+    const int hl = r & 1;
+    r >>= 1;
+    const int mt = r & 1, ks = r >> 1;
+    const int m = lane & 31, g = lane >> 5;
+    const int o = 4 * mt + (m >> 3), s = (m >> 1) & 3, ri = m & 1;
+    const int t = ks < 4 ? 2 * ks + g : (g == 0 ? 8 : -1);
+    const float sw = ks < 4 ? swe : sws;
+    h8v out;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int s2 = q >> 1, rin = q & 1;
+      float v = 0.f;
+      if (t >= 0) {
+        const float2 w = cm[(t * 8 + o) * 16 + s * 4 + s2];
+        v = ri == 0 ? (rin == 0 ? w.x : -w.y) : (rin == 0 ? w.y : w.x);
+      }
+      v *= sw;
+      const _Float16 h = (_Float16)v;
+      out[q] = hl == 0 ? h : (_Float16)(v - (float)h);
+    }
+    Apk[e] = __builtin_bit_cast(uint4, out);
+  }
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
+  typedef SplitShape<32> SH;
+  constexpr int NH = 8 * 2 * 4 * 2 * 64;  // the second layer's hop-term operands (split packing, terms 0..7)
+  __shared__ uint4 lw[NH + kFirstPackOps];
+  if (a.st && a.st->done) return;  // uniform over the grid
+  for (int k = threadIdx.x; k < NH; k += blockDim.x) lw[k] = a.Ah[k];
+  for (int k = threadIdx.x; k < kFirstPackOps; k += blockDim.x) lw[NH + k] = a.A[k];
+  __syncthreads();
+  const uint4 *lh = lw, *le = lw + NH;
+  const Geom g = a.g;
+  // operand scales: s0 the input's, s1 the first layer's even sites'
+  const float s0 = pow2_scale(__uint_as_float(*a.in_max), 14);
+  const float s1 = pow2_scale(__uint_as_float(*a.l1_max), 14);
+  const float oe = a.winv[0] / s0;    // even sites: E against the input
+  const float oo = a.winvh[0] / s1;   // odd sites: the hop terms against L1
+  const float rs = (a.winv[1] * s1) / (a.winvh[0] * s0);  // the site term's part -> the hop terms' scale
+  const int lane = threadIdx.x & 63, gq = lane >> 5, n = lane & 31;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nchunk = (g.vh + 31) / 32;
+  const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
+  const unsigned chan_off = (unsigned)(2 * field);  // one channel (both parities)
+  const char *in1 = a.in + field;  // the input's odd sites
+  float omax = 0.f;
+  f16v acc[3][2];
+  auto clear = [&]() {
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) acc[cc][0] = acc[cc][1] = (f16v){};
+  };
+  // products of one K step: the lane's (scaled) 4 spins x 3 colours, split,
+  // against the two M tiles at A (tile mt at A + mt * mstride, lo at + 64)
+  auto products = [&](const cf (&v)[4][3], const uint4 *A, int mstride) {
+    unsigned uh[3][4], ul[3][4];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) split2(v[s2][cc].re, v[s2][cc].im, uh[cc][s2], ul[cc][s2]);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const h8v ah = as_h8(A[mt * mstride + lane]), al = as_h8(A[mt * mstride + 64 + lane]);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const h8v bh = as_h8(make_uint4(uh[cc][0], uh[cc][1], uh[cc][2], uh[cc][3]));
+        const h8v bl = as_h8(make_uint4(ul[cc][0], ul[cc][1], ul[cc][2], ul[cc][3]));
+        acc[cc][mt] = SH::mfma(ah, bh, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(ah, bl, acc[cc][mt]);
+        acc[cc][mt] = SH::mfma(al, bh, acc[cc][mt]);
+      }
+    }
+  };
+  auto unpack = [](const float4 (&q)[6], cf (&v)[4][3]) {
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int ci = 3 * s2 + cc;
+        const float4 f = q[ci >> 1];
+        v[s2][cc] = (ci & 1) ? mk(f.z, f.w) : mk(f.x, f.y);
+      }
+  };
+  // lane (n, gq) holds spins 2 gq, 2 gq + 1 of the 8 output channels of its
+  // site: float4 planes 3 gq .. 3 gq + 2 of each channel
+  auto store = [&](int i, int p, float os) {
+    const unsigned off = (unsigned)i * 16u;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int o = 4 * mt + j;
+        char *ob = a.out + ((size_t)o * 2 + p) * field;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int l0 = 2 * q, l1 = 2 * q + 1;
+          const int e0 = 4 * j + 2 * (l0 / 3), e1 = 4 * j + 2 * (l1 / 3);
+          float4 f;
+          f.x = acc[l0 % 3][mt][e0] * os;
+          f.y = acc[l0 % 3][mt][e0 + 1] * os;
+          f.z = acc[l1 % 3][mt][e1] * os;
+          f.w = acc[l1 % 3][mt][e1 + 1] * os;
+          omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+          st_at<float4>(ob, (size_t)(3 * gq + q) * plane, off, f);
+        }
+      }
+  };
+  // the one-channel input at odd site j
+  auto load_in = [&](float4 (&q)[6], int j) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(in1, k * plane, (unsigned)j * 16u);
+  };
+  // the first layer's channels 2 ks + gq at the term's (even) neighbour
+  auto load_l1 = [&](float4 (&q)[6], const TermGeo &t, int ks) {
+    const char *cb = a.l1 + ((size_t)(4 * ks) + t.jp) * field;
+    const unsigned off = (unsigned)t.j * 16u + (unsigned)gq * chan_off;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(cb, k * plane, off);
+  };
+  auto load_link = [&](const TermGeo &t, float4 (&l)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      l[k] = ld_at<float4>(a.U.a, ((size_t)t.slab * 3 + k) * plane, (unsigned)t.li * 16u);
+  };
+  int i = 0;
+  bool ok = false;
+  Site se{};
+  TermGeo tg{};
+  float4 lraw[3], vb[6];
+  // chunk c: the lane's site (lanes past the last site compute a copy of a
+  // valid site and store nothing) and the even pass's first loads
+  auto begin = [&](int c) {
+    const int base = c * 32;
+    i = base + n;
+    ok = i < g.vh;
+    if (!ok) i = base;
+    se = decode(g, i, 0);
+    tg = term_geo_u(g, se, i, 0, term_const(g, gq));
+    load_link(tg, lraw);
+    load_in(vb, tg.j);
+  };
+  const int cstep = gridDim.x * WAVES;
+  int c = blockIdx.x * WAVES + wave;
+  if (c < nchunk) begin(c);
+#pragma unroll 1
+  for (; c < nchunk; c += cstep) {
+    const int ic = i;
+    const bool okc = ok;
+    // even output sites: K step ks = the hops in direction ks of the input
+    // (forward on lane group 0, backward on group 1) against E
+    clear();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      cf m[9], v[4][3];
+      term_matrix(lraw, 1, tg.fwd, tg.ph * s0, m);
+      unpack(vb, v);
+      transport4(m, v);
+      __builtin_amdgcn_sched_barrier(0);
+      if (ks < 3) {
+        tg = term_geo_u(g, se, ic, 0, term_const(g, 2 * (ks + 1) + gq));
+        load_link(tg, lraw);
+        load_in(vb, tg.j);
+      } else {
+        load_in(vb, ic);  // the odd pass's site term: the site's own input
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      products(v, le + (ks * 2) * 2 * 64, 2 * 64);
+    }
+    if (okc) store(ic, 0, oe);
+    // odd output sites: the site term S on lane group 0 first, then the
+    // accumulators take the hop terms' scale and the 8 hop terms follow
+    // against the first layer's even sites (the term's link and the next K
+    // step's channels loaded one step ahead, as in k_net_layer_split)
+    const Site so = decode(g, ic, 1);
+    clear();
+    {
+      cf v[4][3];
+      unpack(vb, v);
+      __builtin_amdgcn_sched_barrier(0);
+      tg = term_geo(g, so, ic, 1, 0);
+      load_link(tg, lraw);
+      load_l1(vb, tg, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      const float f = gq == 0 ? s0 : 0.f;
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) v[s2][cc] = f * v[s2][cc];
+      products(v, le + (4 * 2) * 2 * 64, 2 * 64);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        acc[cc][0] *= rs;
+        acc[cc][1] *= rs;
+      }
+    }
+#pragma unroll 1
+    for (int t = 0; t < 8; ++t) {
+      cf m[9];
+      term_matrix(lraw, 1, tg.fwd, tg.ph * s1, m);
+      TermGeo ng = tg;
+      if (t + 1 < 8) {
+        ng = term_geo(g, so, ic, 1, t + 1);
+        load_link(ng, lraw);
+      }
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        cf v[4][3];
+        unpack(vb, v);
+        transport4(m, v);
+        __builtin_amdgcn_sched_barrier(0);
+        if (ks + 1 < 4) load_l1(vb, tg, ks + 1);
+        else if (t + 1 < 8) load_l1(vb, ng, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        products(v, lh + ((t * 2 * 4 + ks) * 2) * 64, 4 * 2 * 64);
+      }
+      tg = ng;
+    }
+    if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads, ahead of the stores
+    if (okc) store(ic, 1, oo);
+  }
+  if (a.out_max) record_max(a.out_max, omax);
+}
+
+int net_pack_second(const float2 *W1, const float2 *W2, void *Apk, float *winv, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_pack_second, dim3(1), dim3(256), 0, s, W1, W2, (uint4 *)Apk, winv);
+  QCU_HIP(hipGetLastError());
+  return 0;
+}
+
+int net_layer_second(const NetSecondArgs &a, hipStream_t s) {
+  QCU_CHECK((size_t)a.g.vh * 96 * 2 + (size_t)a.g.vh * 16 < ((size_t)1 << 32), QCU_ERR_DIMS,
+            "net: lattice too large for the second layer");
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    QCU_HIP(hipGetDevice(&dev));
+    QCU_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu <= 0) ncu = 256;
+  }
+  // 8 waves per block, one block per CU (the operands fill 148 KiB of its LDS)
+  constexpr int waves = 8;
+  const int nchunk = (a.g.vh + 31) / 32;
+  int nblk = ncu;
+  if (nchunk < nblk * waves) nblk = (nchunk + waves - 1) / waves;
+  hipLaunchKernelGGL((k_net_second<waves>), dim3(nblk), dim3(64 * waves), 0, s, a);
   QCU_HIP(hipGetLastError());
   return 0;
 }

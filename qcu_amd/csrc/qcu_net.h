@@ -215,7 +215,29 @@ struct NetFirstArgs {
   const SysState *st;       // early-out flag of a captured solver iteration (or null)
 };
 int net_pack_first(const float2 *W, void *Apk, float *winv, hipStream_t s);
-int net_layer_first(const NetFirstArgs &a, hipStream_t s);
+int net_layer_first(const NetFirstArgs &a, hipStream_t s, bool even_only = false);
+
+// the second layer (8 -> 8) after a first layer on an input zero on the even
+// sites, with the first layer's odd sites left implicit (net_layer_second):
+// even outputs from the input alone, odd outputs from the first layer's even
+// sites and the input
+struct NetSecondArgs {
+  const char *in;           // the network input (channel 0 of both parities; the odd sites are read)
+  const char *l1;           // the first layer's 8 channels (both parities; the even sites are read)
+  char *out;                // the 8 output channels
+  const uint4 *A;           // combined operands E_t, S (net_pack_second)
+  const float *winv;        // their inverse scales: [0] E, [1] S
+  const uint4 *Ah;          // the second layer's own split operands (net_pack_split; terms 0..7 used)
+  const float *winvh;       // their inverse scale
+  const unsigned *in_max;   // largest |component| of the input (float bits)
+  const unsigned *l1_max;   // that of the first layer's even sites
+  unsigned *out_max;        // receives that of the output (or null)
+  LinkPtr U;                // FP32 links
+  Geom g;
+  const SysState *st;       // early-out flag of a captured solver iteration (or null)
+};
+int net_pack_second(const float2 *W1, const float2 *W2, void *Apk, float *winv, hipStream_t s);
+int net_layer_second(const NetSecondArgs &a, hipStream_t s);
 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
