@@ -2257,8 +2257,11 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   QCU_TRY(net_ready(net));
   hipStream_t s = net.lat->stream;
   const int ci = net.chan[layer], co = net.chan[layer + 1];
-  // a split layer runs on the operands packed from the current weights and
-  // the input maximum of the last forward pass
+  // a full forward pass over both parities first (the preconditioner's pass
+  // leaves some activations implicit): the layer then runs on complete
+  // activations, the operands packed from the current weights and the input
+  // maximum that pass recorded
+  QCU_TRY(net.forward(nullptr, s, false, -1));
   const bool split = net.split_layer(layer, 0);
   if (split && net.packed_ver != net.wver) QCU_TRY(net.pack_split(0, s));
   SplitIO io = net.split_io(layer, 0);
@@ -2295,6 +2298,53 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
     const double per_site = mix + 8.0 * (8.0 * (two_pass ? 1 : ci) * 4 * 9);
     *flops = per_site * 2.0 * net.lat->g.vh;
   }
+  return 0;
+}
+
+// average ms of the 8 -> 8 layer fused with the single-output last layer, as
+// the preconditioner runs it: one application first (the same deterministic
+// input as qcuNetBenchApply), then the layer alone on the activations and
+// field maximum that application left, timed with HIP events on the lattice
+// stream
+int qcuNetBenchFusedLayer(QcuNetHandle h, int iters, double *ms) {
+  clear_error();
+  QCU_CHECK(h && ms && iters > 0, QCU_ERR_ARG, "bad arguments");
+  Net &net = h->net;
+  QCU_TRY(net_ready(net));
+  QCU_CHECK(net.fusable(), QCU_ERR_STATE, "net: this network has no fused last layer");
+  hipStream_t s = net.lat->stream;
+  const int vh = net.lat->g.vh;
+  NetTrainer tr{net, s};
+  const size_t nf = (size_t)vh * 24;
+  hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)ndiv(nf, 256)), dim3(256), 0, s, (float *)tr.odd(0).p, nf);
+  QCU_HIP(hipGetLastError());
+  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
+  bool wrote = false;
+  const SpinorPtr o1 = tr.odd(1);
+  QCU_TRY(net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote));
+  const int k = net.nl - 2;
+  SplitIO io = net.split_io(k, 0);
+  io.out_max = nullptr;
+  io.A2 = net.fpk.p;
+  io.winv2 = (const float *)net.swinv.p + 2 * kNetMaxLayers;
+  io.mix = net.mixb.p;
+  auto launch = [&]() {
+    return net.layer(net.act_ptr(k), net.act_ptr(k + 1), (const float2 *)net.W.p + net.woff[k], 8, 8, 0, 2, nullptr,
+                     s, -1, &io);
+  };
+  QCU_TRY(launch());
+  hipEvent_t e0, e1;
+  QCU_HIP(hipEventCreate(&e0));
+  QCU_HIP(hipEventCreate(&e1));
+  QCU_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < iters; ++r) QCU_TRY(launch());
+  QCU_HIP(hipEventRecord(e1, s));
+  QCU_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  QCU_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / iters;
   return 0;
 }
 

@@ -468,6 +468,13 @@ class LearnedPreconditioner:
         _lib.check(self._lib.qcuNetBenchApply(self.handle, int(iters), C.byref(ms)))
         return ms.value
 
+    def bench_fused_layer(self, iters=10):
+        """ms per launch of the 8 -> 8 layer fused with the last layer, as the
+        preconditioner runs it (HIP events)"""
+        ms = C.c_double()
+        _lib.check(self._lib.qcuNetBenchFusedLayer(self.handle, int(iters), C.byref(ms)))
+        return ms.value
+
     def loss(self, sources) -> float:
         n = len(sources)
         sp = (C.c_void_p * n)(*[_field_ptr(s, self.lattice.n_half, "source").value for s in sources])
