@@ -566,6 +566,7 @@ __global__ void __launch_bounds__(256) k_net_mix(NetMixArgs a) {
 
 struct NetGatherArgs {
   SpinorPtr mix;  // the 9 term fields of k_net_mix
+  SpinorPtr mix2; // TWO: a second set of term fields, added to the first; its site term replaces the first's
   SpinorPtr out;  // the output channel (FP32 channel layout), or with DIRECT an odd vector of precision P
   LinkPtr U;      // FP32 links
   Geom g;
@@ -575,8 +576,10 @@ struct NetGatherArgs {
 
 // one thread per output site: the site term plus the 8 transported hop terms.
 // DIRECT: the odd output goes straight into a vector of precision P (the
-// solver's), instead of channel 0 of the FP32 output block.
-template <int P = P32, bool DIRECT = false>
+// solver's), instead of channel 0 of the FP32 output block.  TWO: the hop
+// terms are the sums of two sets of term fields, the site term is the
+// second set's.
+template <int P = P32, bool DIRECT = false, bool TWO = false>
 __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
   if (a.st && a.st->done) return;
   const Geom g = a.g;
@@ -585,7 +588,7 @@ __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
   if (i >= g.vh) return;
   const Site site = decode(g, i, p);
   cf acc[12];
-  SpinorIO<P32>::load(chan_field(a.mix, 8, p, g.vh), i, acc);
+  SpinorIO<P32>::load(chan_field(TWO ? a.mix2 : a.mix, 8, p, g.vh), i, acc);
 #pragma unroll 2
   for (int t = 0; t < 8; ++t) {
     const TermGeo tg = term_geo(g, site, i, p, t);
@@ -593,6 +596,12 @@ __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
     term_link_matrix(a.U, tg, m);
     cf v[12];
     SpinorIO<P32>::load(chan_field(a.mix, t, tg.jp, g.vh), tg.j, v);
+    if constexpr (TWO) {
+      cf w[12];
+      SpinorIO<P32>::load(chan_field(a.mix2, t, tg.jp, g.vh), tg.j, w);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = v[k] + w[k];
+    }
     transport_spins(m, v);
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = acc[k] + v[k];
@@ -1351,6 +1360,9 @@ struct Net : Precond {
   DevBuf swinv;                 // [direction][layer] inverse weight scale, then the fused layer's
   DevBuf fpk;                   // packed operands of the fused single-output layer
   DevBuf p2pk;                  // combined operands of the second layer (net_pack_second), then 2 inverse scales
+  DevBuf mixb2;                 // the second layer's share of the last layer's term fields (third_split)
+  DevBuf wcomp;                 // W4_8 composed with W3 (net_compose_last)
+  DevBuf fpk2;                  // its packed operands (net_pack_fuse), then its inverse scale
   unsigned long long wver = 1;  // weight version (bumped on every change of W)
   unsigned long long packed_ver = 0;
   size_t field_bytes() const { return (size_t)lat->g.vh * 2 * 96; }  // one channel, both parities
@@ -1379,6 +1391,12 @@ struct Net : Precond {
   // layer's odd sites left implicit, when only the network output is needed
   bool second_split() const { return first_split() && nl >= 3 && chan[2] == 8 && split_layer(1, 0) && split_ns == 32; }
   float *p2winv() const { return (float *)((char *)p2pk.p + (size_t)kFirstPackOps * 16); }
+  // ... and, in a 1-8-8-8-1 network, the third layer on its even output sites
+  // only: its odd outputs feed only the last layer's site term, whose
+  // composition with the third layer (W4_8 W3_u) the second layer applies in
+  // its epilogue, writing mixb2
+  bool third_split() const { return second_split() && nl == 4 && fusable() && mixb2.p != nullptr; }
+  float *fpk2winv() const { return (float *)((char *)fpk2.p + kFusePackBytes); }
   bool input_fresh = false;  // load_input ran since the last forward pass: maxima cleared, input's recorded
   // act[0] <- the odd vector in (precision prec) as channel 0, zero on the
   // even sites; clears the field maxima of the next forward pass first
@@ -1390,8 +1408,10 @@ struct Net : Precond {
            net_split_can_fuse(split_ns) && mixb.p != nullptr;
   }
   // the second pass of the single-output layer: transports and sums the term fields in mixb
-  // direct_prec >= 0: out is an odd vector of that precision (the odd output parity only)
-  int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec = -1);
+  // direct_prec >= 0: out is an odd vector of that precision (the odd output parity only);
+  // two: the term fields are the sums of mixb and mixb2 (site term: mixb2's)
+  int gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec = -1,
+             bool two = false);
 
   int init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init_scale);
   SpinorPtr act_ptr(int layer) const {
@@ -1501,6 +1521,8 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   QCU_TRY(swinv.alloc((size_t)(2 * kNetMaxLayers + 1) * 4));
   QCU_TRY(fpk.alloc(kFusePackBytes));
   QCU_TRY(p2pk.alloc((size_t)kFirstPackOps * 16 + 16));
+  QCU_TRY(wcomp.alloc((size_t)9 * 8 * 16 * sizeof(float2)));
+  QCU_TRY(fpk2.alloc(kFusePackBytes + 16));
   QCU_HIP(hipMemset(smax.p, 0, (size_t)2 * (kNetMaxLayers + 1) * 4));
   if (const char *e = getenv("QCU_NET_SPLIT")) use_split = atoi(e) != 0;
   if (const char *e = getenv("QCU_NET_SPLIT_ORDER")) split_order = atoi(e);
@@ -1514,6 +1536,10 @@ int Net::init(Lattice *l, int nlayers, const int *ch, uint64_t seed, double init
   int cbig = 1;
   for (int k = 0; k <= nl; ++k) cbig = std::max(cbig, chan[k]);
   if (use_mfma && cbig > 1) QCU_TRY(mixb.alloc(9 * field_bytes()));
+  // the second set of term fields of a 1-8-8-8-1 preconditioner
+  if (use_mfma && use_split && nl == 4 && chan[0] == 1 && chan[1] == 8 && chan[2] == 8 && chan[3] == 8 &&
+      chan[4] == 1)
+    QCU_TRY(mixb2.alloc(9 * field_bytes()));
   // launch shape of the layer kernel: QCU_NET_SHAPE="co,block" overrides it
   if (const char *e = getenv("QCU_NET_SHAPE")) {
     int c = 0, b = 0;
@@ -1645,11 +1671,15 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
 }
 
 // act[0] holds the input; fills act[1..nl] (act[nl] = output)
-int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec) {
+int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hipStream_t s, int direct_prec,
+                bool two) {
   NetGatherArgs ga;
   ga.mix.p = mixb.p;
   ga.mix.norm = nullptr;
   ga.mix.stride = lat->g.vh;
+  ga.mix2 = ga.mix;
+  ga.mix2.p = mixb2.p;
+  QCU_CHECK(!two || mixb2.p, QCU_ERR_STATE, "net: second term-field set not allocated");
   ga.out = out;
   ga.U = lat->links(P32);
   ga.g = lat->g;
@@ -1657,11 +1687,20 @@ int Net::gather(SpinorPtr out, int parity0, int nparity, const SysState *st, hip
   ga.st = st;
   const dim3 grid((unsigned)ndiv(lat->g.vh, 256), nparity);
   QCU_CHECK(direct_prec < 0 || (parity0 == 1 && nparity == 1), QCU_ERR_ARG, "net: direct output is the odd parity");
-  switch (direct_prec) {
-    case P64: hipLaunchKernelGGL((k_net_gather<P64, true>), grid, dim3(256), 0, s, ga); break;
-    case P32: hipLaunchKernelGGL((k_net_gather<P32, true>), grid, dim3(256), 0, s, ga); break;
-    case P16: hipLaunchKernelGGL((k_net_gather<P16, true>), grid, dim3(256), 0, s, ga); break;
-    default: hipLaunchKernelGGL((k_net_gather<P32, false>), grid, dim3(256), 0, s, ga); break;
+  if (two) {
+    switch (direct_prec) {
+      case P64: hipLaunchKernelGGL((k_net_gather<P64, true, true>), grid, dim3(256), 0, s, ga); break;
+      case P32: hipLaunchKernelGGL((k_net_gather<P32, true, true>), grid, dim3(256), 0, s, ga); break;
+      case P16: hipLaunchKernelGGL((k_net_gather<P16, true, true>), grid, dim3(256), 0, s, ga); break;
+      default: hipLaunchKernelGGL((k_net_gather<P32, false, true>), grid, dim3(256), 0, s, ga); break;
+    }
+  } else {
+    switch (direct_prec) {
+      case P64: hipLaunchKernelGGL((k_net_gather<P64, true>), grid, dim3(256), 0, s, ga); break;
+      case P32: hipLaunchKernelGGL((k_net_gather<P32, true>), grid, dim3(256), 0, s, ga); break;
+      case P16: hipLaunchKernelGGL((k_net_gather<P16, true>), grid, dim3(256), 0, s, ga); break;
+      default: hipLaunchKernelGGL((k_net_gather<P32, false>), grid, dim3(256), 0, s, ga); break;
+    }
   }
   QCU_HIP(hipGetLastError());
   return 0;
@@ -1676,6 +1715,10 @@ int Net::pack_split(int dir, hipStream_t s) {
   }
   if (dir == 0 && second_split())
     QCU_TRY(net_pack_second((const float2 *)W.p + woff[0], (const float2 *)W.p + woff[1], p2pk.p, p2winv(), s));
+  if (dir == 0 && third_split()) {
+    QCU_TRY(net_compose_last((const float2 *)W.p + woff[2], (const float2 *)W.p + woff[3], (float2 *)wcomp.p, s));
+    QCU_TRY(net_pack_fuse((const float2 *)wcomp.p, fpk2.p, fpk2winv(), s));
+  }
   for (int k = 0; k < nl; ++k) {
     if (!split_layer(k, dir)) continue;
     const SplitIO io = split_io(k, dir);
@@ -1720,15 +1763,18 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
   const bool fuse = !keep_acts && odd_only_last && fusable();
   // the first two layers as net_layer_first (even sites) + net_layer_second
   const bool second = first && !keep_acts && second_split() && !(fuse && nl - 2 == 1);
+  const bool third = second && fuse && third_split();
   for (int k = 0; k < nl; ++k) {
     const bool last = k == nl - 1;
-    const int p0 = (last && odd_only_last) ? 1 : 0, np = (last && odd_only_last) ? 1 : 2;
+    const int p0 = (last && odd_only_last) ? 1 : 0;
+    // third: the fused layer on its even output sites only
+    const int np = (last && odd_only_last) || (third && k == nl - 2) ? 1 : 2;
     if (fuse && last) {
       if (direct) {
-        QCU_TRY(gather(*direct, p0, np, st, s, direct_prec));
+        QCU_TRY(gather(*direct, p0, np, st, s, direct_prec, third));
         if (wrote) *wrote = true;
       } else {
-        QCU_TRY(gather(act_ptr(k + 1), p0, np, st, s));
+        QCU_TRY(gather(act_ptr(k + 1), p0, np, st, s, -1, third));
       }
       break;
     }
@@ -1770,6 +1816,11 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
       q.U = lat->links(P32);
       q.g = lat->g;
       q.st = st;
+      if (third) {
+        q.A2 = (const uint4 *)fpk2.p;
+        q.winv2 = fpk2winv();
+        q.mix = (char *)mixb2.p;
+      }
       QCU_TRY(net_layer_second(q, s));
       continue;
     }
@@ -2328,8 +2379,10 @@ int qcuNetBenchFusedLayer(QcuNetHandle h, int iters, double *ms) {
   io.A2 = net.fpk.p;
   io.winv2 = (const float *)net.swinv.p + 2 * kNetMaxLayers;
   io.mix = net.mixb.p;
+  // a 1-8-8-8-1 preconditioner runs this layer on its even output sites only
+  const int np = net.third_split() ? 1 : 2;
   auto launch = [&]() {
-    return net.layer(net.act_ptr(k), net.act_ptr(k + 1), (const float2 *)net.W.p + net.woff[k], 8, 8, 0, 2, nullptr,
+    return net.layer(net.act_ptr(k), net.act_ptr(k + 1), (const float2 *)net.W.p + net.woff[k], 8, 8, 0, np, nullptr,
                      s, -1, &io);
   };
   QCU_TRY(launch());

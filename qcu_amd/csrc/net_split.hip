@@ -187,11 +187,20 @@ QCU_D void transport4(const cf (&m)[9], cf (&v)[4][3]) {
 // from their largest component (the two lanes of a site share it) before the
 // FP16 split.  Even output sites produce the 8 hop-term fields (two M tiles),
 // odd ones the site-term field (one tile, rows 0..7).
+// Operands and output of the single-output mixing: selections 0 and 1 (the
+// hop terms) at A01, in LDS or global memory, selection 2 (the site term) at
+// A2 in global memory; winv2 their inverse scale; mix the term fields.
+struct MixOut {
+  const uint4 *A01, *A2;
+  const float *winv2;
+  char *mix;
+};
+
 // pre() is called once the accumulators are consumed: the caller issues its
 // next loads there.
 template <class Pre>
-QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&acc)[3][2], float oscale, int i,
-                          int p, bool ok, int g, int lane, size_t field, size_t plane, Pre pre) {
+QCU_D void fused_epilogue(const MixOut &mo, const f16v (&acc)[3][2], float oscale, int i, int p, bool ok, int g,
+                          int lane, size_t field, size_t plane, Pre pre) {
   float mx = 0.f;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc)
@@ -202,7 +211,7 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&
   mx = fmaxf(mx, __shfl_xor(mx, 32));
   const float sn = pow2_scale(mx * oscale, 14);
   const float f = oscale * sn;                 // accumulator -> scaled B operand (a power of two)
-  const float yscale = a.winv2[0] / sn;        // mixed field -> its true value
+  const float yscale = mo.winv2[0] / sn;       // mixed field -> its true value
   const unsigned off = (unsigned)i * 16u;
   // the B operands of the mixing, shared by its selections: per K step and
   // colour, the accumulators' rows of that K step, scaled and split
@@ -228,8 +237,7 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&
   pre();
   __builtin_amdgcn_sched_barrier(0);
   // selection sel: one M tile of the mixing (8 packed operands of 64 lanes:
-  // K step ks, hi / lo); the hop-term operands (selections 0 and 1) are in
-  // LDS, the site term's in global memory (the layer's weights leave no room)
+  // K step ks, hi / lo)
   const int nsel = p == 0 ? 2 : 1;
 #pragma unroll 1
   for (int k = 0; k < nsel; ++k) {
@@ -241,11 +249,11 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&
     for (int ks = 0; ks < 4; ++ks) {
       h8v ah, al;
       if (p == 0) {
-        ah = as_h8(lA2[((k * 4 + ks) * 2) * 64 + lane]);
-        al = as_h8(lA2[((k * 4 + ks) * 2 + 1) * 64 + lane]);
+        ah = as_h8(mo.A01[((k * 4 + ks) * 2) * 64 + lane]);
+        al = as_h8(mo.A01[((k * 4 + ks) * 2 + 1) * 64 + lane]);
       } else {
-        ah = as_h8(a.A2[((8 + ks) * 2) * 64 + lane]);
-        al = as_h8(a.A2[((8 + ks) * 2 + 1) * 64 + lane]);
+        ah = as_h8(mo.A2[((8 + ks) * 2) * 64 + lane]);
+        al = as_h8(mo.A2[((8 + ks) * 2 + 1) * 64 + lane]);
       }
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
@@ -261,7 +269,7 @@ QCU_D void fused_epilogue(const NetSplitArgs &a, const uint4 *lA2, const f16v (&
     for (int tl = 0; tl < 4; ++tl) {
       if (sel == 2 && tl > 0) break;
       const int t = sel == 2 ? 8 : 4 * sel + tl;
-      char *ob = a.mix + ((size_t)t * 2 + p) * field;
+      char *ob = mo.mix + ((size_t)t * 2 + p) * field;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const int l0 = 2 * q, l1 = 2 * q + 1;
@@ -438,7 +446,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   // once the accumulators are split in the fused epilogue
   auto finish = [&](const Chunk &k, auto pre) {
     if constexpr (FUSE) {
-      fused_epilogue(a, lw + NA, acc, oscale, k.i, k.p, k.ok, gq, lane, field, plane, pre);
+      const MixOut mo{lw + NA, a.A2, a.winv2, a.mix};
+      fused_epilogue(mo, acc, oscale, k.i, k.p, k.ok, gq, lane, field, plane, pre);
     } else {
       pre();
       if (!k.ok) return;
@@ -900,7 +909,9 @@ __global__ void __launch_bounds__(256) k_net_pack_second(const float2 *W1, const
   }
 }
 
-template <int WAVES>
+// MIX: the epilogue also writes the next layer's single-output term fields
+// (fused_epilogue with operands in global memory)
+template <int WAVES, bool MIX>
 __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
   typedef SplitShape<32> SH;
   constexpr int NH = 8 * 2 * 4 * 2 * 64;  // the second layer's hop-term operands (split packing, terms 0..7)
@@ -1039,13 +1050,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
         tg = term_geo_u(g, se, ic, 0, term_const(g, 2 * (ks + 1) + gq));
         load_link(tg, lraw);
         load_in(vb, tg.j);
-      } else {
+      } else if constexpr (!MIX) {
         load_in(vb, ic);  // the odd pass's site term: the site's own input
       }
       __builtin_amdgcn_sched_barrier(0);
       products(v, le + (ks * 2) * 2 * 64, 2 * 64);
     }
     if (okc) store(ic, 0, oe);
+    if constexpr (MIX) {
+      // the hop-term fields of the single-output mixing; the odd pass's site
+      // operand is loaded once the accumulators are split
+      const MixOut mo{a.A2, a.A2, a.winv2, a.mix};
+      fused_epilogue(mo, acc, oe, ic, 0, okc, gq, lane, field, plane, [&]() { load_in(vb, ic); });
+    }
     // odd output sites: the site term S on lane group 0 first, then the
     // accumulators take the hop terms' scale and the 8 hop terms follow
     // against the first layer's even sites (the term's link and the next K
@@ -1094,10 +1111,44 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
       }
       tg = ng;
     }
-    if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads, ahead of the stores
-    if (okc) store(ic, 1, oo);
+    if constexpr (MIX) {
+      if (okc) store(ic, 1, oo);
+      // the site-term field; the next chunk's first loads once the accumulators are split
+      const MixOut mo{a.A2, a.A2, a.winv2, a.mix};
+      fused_epilogue(mo, acc, oo, ic, 1, okc, gq, lane, field, plane, [&]() {
+        if (c + cstep < nchunk) begin(c + cstep);
+      });
+    } else {
+      if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads, ahead of the stores
+      if (okc) store(ic, 1, oo);
+    }
   }
   if (a.out_max) record_max(a.out_max, omax);
+}
+
+// The last layer's site term composed with the layer before it:
+// Wc[u][0][i] = W4_8[0][j] W3_u[j][i] (sum over j and the inner spin), the
+// weights of a single-output layer over that layer's input (u = 0..8), laid
+// out as a (cin 8, cout 1) layer for net_pack_fuse.  FP64 sums, FP32 result.
+__global__ void __launch_bounds__(256) k_net_compose_last(const float2 *W3, const float2 *W4, float2 *Wc) {
+  for (int e = threadIdx.x; e < 9 * 8 * 16; e += blockDim.x) {
+    const int u = e >> 7, i = (e >> 4) & 7, s = (e >> 2) & 3, s2 = e & 3;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < 8; ++j)
+      for (int sp = 0; sp < 4; ++sp) {
+        const float2 a = W4[((size_t)8 * 8 + j) * 16 + s * 4 + sp];
+        const float2 b = W3[(((size_t)u * 8 + j) * 8 + i) * 16 + sp * 4 + s2];
+        re += (double)a.x * b.x - (double)a.y * b.y;
+        im += (double)a.x * b.y + (double)a.y * b.x;
+      }
+    Wc[e] = make_float2((float)re, (float)im);
+  }
+}
+
+int net_compose_last(const float2 *W3, const float2 *W4, float2 *Wc, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_compose_last, dim3(1), dim3(256), 0, s, W3, W4, Wc);
+  QCU_HIP(hipGetLastError());
+  return 0;
 }
 
 int net_pack_second(const float2 *W1, const float2 *W2, void *Apk, float *winv, hipStream_t s) {
@@ -1121,7 +1172,8 @@ int net_layer_second(const NetSecondArgs &a, hipStream_t s) {
   const int nchunk = (a.g.vh + 31) / 32;
   int nblk = ncu;
   if (nchunk < nblk * waves) nblk = (nchunk + waves - 1) / waves;
-  hipLaunchKernelGGL((k_net_second<waves>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  if (a.A2) hipLaunchKernelGGL((k_net_second<waves, true>), dim3(nblk), dim3(64 * waves), 0, s, a);
+  else hipLaunchKernelGGL((k_net_second<waves, false>), dim3(nblk), dim3(64 * waves), 0, s, a);
   QCU_HIP(hipGetLastError());
   return 0;
 }

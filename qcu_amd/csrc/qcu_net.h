@@ -235,8 +235,16 @@ struct NetSecondArgs {
   LinkPtr U;                // FP32 links
   Geom g;
   const SysState *st;       // early-out flag of a captured solver iteration (or null)
+  // optional: the last layer's site-term mixing composed with the next layer,
+  // written as the single-output layer's term fields (hop terms at even
+  // sites, the site term at odd ones; see NetSplitArgs); null A2: none
+  const uint4 *A2 = nullptr;
+  const float *winv2 = nullptr;
+  char *mix = nullptr;
 };
 int net_pack_second(const float2 *W1, const float2 *W2, void *Apk, float *winv, hipStream_t s);
+// Wc[u][0][i][4x4] = W4_8[0][j] W3_u[j][i] (8 -> 8 layer W3, 8 -> 1 layer W4; u = 0..8)
+int net_compose_last(const float2 *W3, const float2 *W4, float2 *Wc, hipStream_t s);
 int net_layer_second(const NetSecondArgs &a, hipStream_t s);
 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
