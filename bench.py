@@ -42,7 +42,7 @@ COMPULSORY_HOP_BYTES = {0: 192 + 8 * 96 + 576 + 192 + 192, 1: 96 + 8 * 48 + 288 
                         2: 52 + 8 * 24 + 148 + 52 + 52}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp16_clover_32x32x32x64_traffic.json")
 NET_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_split_32x32x32x64_traffic.json")
-FUSED_TRAFFIC = os.path.join(ROOT, "profiles", "net_layer_8x8_fused_32x32x32x64_traffic.json")
+SECOND_TRAFFIC = os.path.join(ROOT, "profiles", "net_second_32x32x32x64_traffic.json")
 HOP64_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_clover_32x32x32x64_traffic.json")
 HOP64W_TRAFFIC = os.path.join(ROOT, "profiles", "hop_fp64_wilson_32x32x32x64_traffic.json")
 
@@ -204,19 +204,25 @@ def main():
     assert all(r.converged and r.true_res <= TOL for r in lres), "learned solve did not reach 1e-10"
     log(f"learned+mixed: {t_learned * 1e3:.1f} ms {lres[-1]}")
 
-    # roofline of the dominant kernel of the timed solve: the network's third
-    # layer (8 -> 8) with the last layer's 8 -> 1 mixing fused into its
-    # epilogue, on the FP16 matrix cores with split FP32 operands
-    # (k_net_layer_split<32, 8, 2, true>), timed with HIP events on the stream
-    # it runs on as the preconditioner runs it.  Its bound is memory: the
-    # compulsory bytes are the 8 input channels (both parities), the FP32 links
-    # once and the 9 mixed term fields it writes (hop terms at even sites, the
-    # site term at odd ones); the measured traffic (profiles/) shows the
-    # neighbour channels re-read past L2.
-    ms_fused = net.bench_fused_layer(iters=10)
-    fused_bytes = 8 * 2 * vh * 96 + 8 * vh * 48 + 9 * vh * 96
-    fused_GBps = fused_bytes / (ms_fused * 1e-3) / 1e9
-    fused_traffic = traffic_of(FUSED_TRAFFIC)
+    # roofline of the dominant kernel of the timed solve: the network's second
+    # layer as the preconditioner runs it (k_net_second<8, true>): its even
+    # outputs from the one-channel input with combined matrices, its odd
+    # outputs from the first layer's even sites and the input, and in its
+    # epilogue the last layer's term fields for the third layer's odd sites;
+    # split FP16 on the matrix cores, timed with HIP events on the stream it
+    # runs on.  Its bound is memory: the compulsory bytes are the input's odd
+    # sites, the first layer's 8 channels at even sites, the FP32 links once,
+    # its 8 output channels (both parities) and the 9 term fields it writes;
+    # the measured traffic (profiles/) shows the neighbour channels re-read.
+    stages = [net.bench_precond_layer(k, iters=10) for k in range(4)]
+    ms_second = stages[1]
+    second_bytes = (96 + 8 * 96 + 8 * 48 + 2 * 8 * 96 + 9 * 96) * vh
+    second_GBps = second_bytes / (ms_second * 1e-3) / 1e9
+    second_traffic = traffic_of(SECOND_TRAFFIC)
+    # executed v_mfma_f32_32x32x16_f16 (32768 flops each) per 32 sites of
+    # both parities: even pass 4 K steps, odd pass 33, 6 products each per
+    # M tile pair x 3 colours (18 per K step), epilogue 72 + 36
+    second_mfma_flops = (4 * 18 + 33 * 18 + 72 + 36) * 32768 // 32 * vh
     # the plain 8 -> 8 layer (training's forward pass; every 8 -> 8 layer outside the preconditioner)
     ms_layer, fl_layer = net.bench_layer(1, iters=10)
     layer_bytes = 8 * 2 * vh * 96 * 2 + 8 * vh * 48
@@ -226,9 +232,6 @@ def main():
     # complex mixing, 9 terms x 8 x 8 channels x 16 spin pairs x 3 colours x
     # 4 real multiply-adds x 2 flops per site
     mfma_flops = 3 * 9 * 8 * 8 * 16 * 3 * 4 * 2 * 2 * vh
-    # the fused epilogue adds 72 (even sites) and 36 (odd sites)
-    # v_mfma_f32_32x32x16_f16 per 32 sites, 32768 flops each
-    fused_mfma_flops = mfma_flops + (72 + 36) * 32768 // 32 * vh
     ms_apply = net.bench_apply(iters=10)
     # the inner-precision clover hop with the xpay epilogue (HBM-bound)
     ms_hop = lat.bench_hop(prec=inner, clover=True, xpay=True, iters=100)
@@ -288,19 +291,21 @@ def main():
                      "pairs": "plain FP64 BiCGSTAB to 1e-8" if sol_pairs else None,
                      "epochs": args.train_epochs, "batch": args.train_batch,
                      "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])},
-        "roofline": {"kernel": "network layer 8->8 channels with the last layer's mixing fused "
-                               "(k_net_layer_split<32, 8, 2, true>)", "bound": "hbm",
+        "roofline": {"kernel": "network layer 2 as the preconditioner runs it (k_net_second<8, true>)",
+                     "bound": "hbm",
                      "compute": "v_mfma_f32_32x32x16_f16, FP32 operands split into FP16 hi + lo, three products "
                                 "accumulated in FP32 (FP32-accurate)",
-                     "bytes_count": "compulsory: 8 input channels (FP32, both parities) + FP32 links once + the 9 "
-                                    "mixed term fields written (hop terms at even sites, the site term at odd)",
-                     "achieved": round(fused_GBps, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fused_GBps / PEAK_GBS, 3), "traffic": fused_traffic,
-                     "traffic_GBps": None if fused_traffic is None else
-                     round(fused_traffic / (ms_fused * 1e-3) / 1e9, 1),
-                     "ms_per_launch": round(ms_fused, 4), "alg_bytes_per_launch": fused_bytes,
-                     "fp16_mfma_TFLOPs": round(fused_mfma_flops / (ms_fused * 1e-3) / 1e12, 1),
-                     "fp16_mfma_frac": round(fused_mfma_flops / (ms_fused * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 3)},
+                     "bytes_count": "compulsory: the input's odd sites, layer 1's 8 channels at even sites, FP32 "
+                                    "links once, 8 output channels (both parities), the 9 term fields written",
+                     "achieved": round(second_GBps, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(second_GBps / PEAK_GBS, 3), "traffic": second_traffic,
+                     "traffic_GBps": None if second_traffic is None else
+                     round(second_traffic / (ms_second * 1e-3) / 1e9, 1),
+                     "ms_per_launch": round(ms_second, 4), "alg_bytes_per_launch": second_bytes,
+                     "fp16_mfma_TFLOPs": round(second_mfma_flops / (ms_second * 1e-3) / 1e12, 1),
+                     "fp16_mfma_frac": round(second_mfma_flops / (ms_second * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 3)},
+        "precond_layer_ms": {"layer1_first": round(stages[0], 4), "layer2_second": round(stages[1], 4),
+                             "layer3_fused_even_sites": round(stages[2], 4), "layer4_gather": round(stages[3], 4)},
         "layer_8x8_plain": {"kernel": "network layer 8->8 channels (k_net_layer_split<32, 8, 2>)", "bound": "hbm",
                      "compute": "v_mfma_f32_32x32x16_f16, FP32 operands split into FP16 hi + lo, three products "
                                 "accumulated in FP32 (FP32-accurate)",

@@ -229,11 +229,10 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
  * field embedded, forward pass keeping only the output, odd part extracted;
  * no counterpart in the reference) */
 int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms);
-/* average ms of the 8 -> 8 layer fused with the single-output last layer, as
- * the preconditioner runs it (after one application, on its activations);
- * QCU_ERR_STATE if the network has no such layer; no counterpart in the
- * reference */
-int qcuNetBenchFusedLayer(QcuNetHandle h, int iters, double *ms);
+/* average ms of one layer's kernels as the preconditioner runs them (after
+ * one application, on its activations; the last layer of a network whose
+ * last layer is fused: its gather pass); no counterpart in the reference */
+int qcuNetBenchPrecondLayer(QcuNetHandle h, int layer, int iters, double *ms);
 /* out = K in on one-parity (odd) external fields, as applied inside the solver */
 int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in);
 /* weights file: one line "QCUNET1 {json metadata}" (channels, weight count,

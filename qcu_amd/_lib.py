@@ -100,7 +100,7 @@ SIGNATURES = {
     "qcuNetLossGrad": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
     "qcuNetBenchLayer": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "qcuNetBenchApply": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
-    "qcuNetBenchFusedLayer": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
+    "qcuNetBenchPrecondLayer": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "qcuNetSave": (C.c_int, [_VP, C.c_char_p]),
     "qcuNetLoad": (C.c_int, [_VP, C.c_char_p]),
 }

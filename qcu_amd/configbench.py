@@ -5,7 +5,7 @@ residual) and the kernel measurements behind the rocprofv3 summaries.
     python -m qcu_amd.configbench hop [--prec P --clover 0/1 --xpay 0/1 --iters N]
     python -m qcu_amd.configbench net-layer [--layer K --iters N]
     python -m qcu_amd.configbench net-apply [--iters N]
-    python -m qcu_amd.configbench net-fused [--iters N]
+    python -m qcu_amd.configbench net-stage [--layer K --iters N]
     python -m qcu_amd.configbench net-backward [--iters N]
     python -m qcu_amd.configbench copy [--mib M --iters N]
 
@@ -37,8 +37,8 @@ net-layer    one forward layer of the 1-8-8-8-1 network (qcuNetBenchLayer) on
              the activations of a forward pass over Gaussian data.
 net-apply    the network applied as the preconditioner (odd field in, odd part
              of the output kept; qcuNetApplyOdd), median wall ms.
-net-fused    the 8 -> 8 layer fused with the last layer, as the preconditioner
-             runs it (qcuNetBenchFusedLayer), ms per launch in HIP events.
+net-stage    one layer's kernels as the preconditioner runs them
+             (qcuNetBenchPrecondLayer), ms per launch in HIP events.
 net-backward full forward + backward passes (qcuNetBackward), median wall ms.
 copy         a device-to-device copy and a fill of M MiB: the plain streams of
              known size that scale the FETCH_SIZE / WRITE_SIZE counters.
@@ -286,10 +286,10 @@ def net_apply(args):
             "net_apply_ms_events": net.bench_apply(iters=args.iters)}
 
 
-def net_fused(args):
+def net_stage(args):
     lat = _headline_lattice(args)
     net = _network(lat)
-    return {"L": lat.L, "ms_per_launch": net.bench_fused_layer(iters=args.iters)}
+    return {"L": lat.L, "layer": args.layer, "ms_per_launch": net.bench_precond_layer(args.layer, iters=args.iters)}
 
 
 def net_backward(args):
@@ -328,7 +328,7 @@ def copy(args):
 def main():
     import argparse
     fns = {"c1": c1, "c2": c2, "c3a": c3a, "c3b": c3b, "c4": c4, "legacy": legacy}
-    kernels = {"hop": hop, "net-layer": net_layer, "net-apply": net_apply, "net-fused": net_fused, "net-backward": net_backward,
+    kernels = {"hop": hop, "net-layer": net_layer, "net-apply": net_apply, "net-stage": net_stage, "net-backward": net_backward,
                "copy": copy}
     ap = argparse.ArgumentParser(description="workload and kernel measurements")
     ap.add_argument("which", nargs="*", default=["c1", "c2", "c3a", "c3b"],

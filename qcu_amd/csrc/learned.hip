@@ -1429,8 +1429,10 @@ struct Net : Precond {
   // keep_acts false: only act[nl] is needed afterwards (the last layer may be fused)
   // direct (with odd_only_last, keep_acts false): the odd output may go straight into this
   // vector of precision direct_prec; *wrote tells whether it did (fused last layer)
+  // only >= 0: run layer `only` alone, on the activations and field maxima a
+  // full pass of the same kind left (timing)
   int forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts = true,
-              const SpinorPtr *direct = nullptr, int direct_prec = P32, bool *wrote = nullptr);
+              const SpinorPtr *direct = nullptr, int direct_prec = P32, bool *wrote = nullptr, int only = -1);
   // gout_zero_par: parity on which gout_last is zero; act0_zero_par: parity on
   // which the network input act[0] is zero (-1: none)
   int backward(SpinorPtr gout_last, hipStream_t s, bool want_grad_in, SpinorPtr grad_in, int gout_zero_par = -1,
@@ -1746,7 +1748,7 @@ int Net::reset_max(int dir, hipStream_t s) {
 }
 
 int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero_par0, bool keep_acts,
-                 const SpinorPtr *direct, int direct_prec, bool *wrote) {
+                 const SpinorPtr *direct, int direct_prec, bool *wrote, int only) {
   if (wrote) *wrote = false;
   bool any = first_split();
   for (int k = 0; k < nl; ++k) any = any || split_layer(k, 0);
@@ -1754,7 +1756,7 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
   const bool first = first_split() && zero_par0 == 0 && input_fresh;
   if (any) {
     if (packed_ver != wver) QCU_TRY(pack_split(0, s));
-    if (!input_fresh) QCU_TRY(reset_max(0, s));
+    if (!input_fresh && only < 0) QCU_TRY(reset_max(0, s));
   }
   input_fresh = false;
   // fused: the 8 -> 8 layer before the last writes the last layer's mixed
@@ -1769,6 +1771,7 @@ int Net::forward(const SysState *st, hipStream_t s, bool odd_only_last, int zero
     const int p0 = (last && odd_only_last) ? 1 : 0;
     // third: the fused layer on its even output sites only
     const int np = (last && odd_only_last) || (third && k == nl - 2) ? 1 : 2;
+    if (only >= 0 && k != only) continue;
     if (fuse && last) {
       if (direct) {
         QCU_TRY(gather(*direct, p0, np, st, s, direct_prec, third));
@@ -2352,17 +2355,17 @@ int qcuNetBenchLayer(QcuNetHandle h, int layer, int iters, double *ms, double *f
   return 0;
 }
 
-// average ms of the 8 -> 8 layer fused with the single-output last layer, as
-// the preconditioner runs it: one application first (the same deterministic
-// input as qcuNetBenchApply), then the layer alone on the activations and
-// field maximum that application left, timed with HIP events on the lattice
-// stream
-int qcuNetBenchFusedLayer(QcuNetHandle h, int iters, double *ms) {
+// average ms of layer `layer`'s kernels as the preconditioner runs them: one
+// application first (the same deterministic input as qcuNetBenchApply), then
+// that layer alone on the activations and field maxima the application left
+// (the last layer of a fused network: its gather pass), timed with HIP events
+// on the lattice stream
+int qcuNetBenchPrecondLayer(QcuNetHandle h, int layer, int iters, double *ms) {
   clear_error();
   QCU_CHECK(h && ms && iters > 0, QCU_ERR_ARG, "bad arguments");
   Net &net = h->net;
   QCU_TRY(net_ready(net));
-  QCU_CHECK(net.fusable(), QCU_ERR_STATE, "net: this network has no fused last layer");
+  QCU_CHECK(layer >= 0 && layer < net.nl, QCU_ERR_ARG, "layer out of range");
   hipStream_t s = net.lat->stream;
   const int vh = net.lat->g.vh;
   NetTrainer tr{net, s};
@@ -2373,17 +2376,10 @@ int qcuNetBenchFusedLayer(QcuNetHandle h, int iters, double *ms) {
   bool wrote = false;
   const SpinorPtr o1 = tr.odd(1);
   QCU_TRY(net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote));
-  const int k = net.nl - 2;
-  SplitIO io = net.split_io(k, 0);
-  io.out_max = nullptr;
-  io.A2 = net.fpk.p;
-  io.winv2 = (const float *)net.swinv.p + 2 * kNetMaxLayers;
-  io.mix = net.mixb.p;
-  // a 1-8-8-8-1 preconditioner runs this layer on its even output sites only
-  const int np = net.third_split() ? 1 : 2;
+  const bool fresh = net.first_split();  // the pass ran the path that needs a recorded input maximum
   auto launch = [&]() {
-    return net.layer(net.act_ptr(k), net.act_ptr(k + 1), (const float2 *)net.W.p + net.woff[k], 8, 8, 0, np, nullptr,
-                     s, -1, &io);
+    net.input_fresh = fresh;
+    return net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote, layer);
   };
   QCU_TRY(launch());
   hipEvent_t e0, e1;

@@ -468,11 +468,11 @@ class LearnedPreconditioner:
         _lib.check(self._lib.qcuNetBenchApply(self.handle, int(iters), C.byref(ms)))
         return ms.value
 
-    def bench_fused_layer(self, iters=10):
-        """ms per launch of the 8 -> 8 layer fused with the last layer, as the
-        preconditioner runs it (HIP events)"""
+    def bench_precond_layer(self, layer, iters=10):
+        """ms per launch of one layer's kernels as the preconditioner runs them
+        (HIP events)"""
         ms = C.c_double()
-        _lib.check(self._lib.qcuNetBenchFusedLayer(self.handle, int(iters), C.byref(ms)))
+        _lib.check(self._lib.qcuNetBenchPrecondLayer(self.handle, int(layer), int(iters), C.byref(ms)))
         return ms.value
 
     def loss(self, sources) -> float:
