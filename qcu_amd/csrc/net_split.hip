@@ -188,8 +188,9 @@ QCU_D void transport4(const cf (&m)[9], cf (&v)[4][3]) {
 // FP16 split.  Even output sites produce the 8 hop-term fields (two M tiles),
 // odd ones the site-term field (one tile, rows 0..7).
 // Operands and output of the single-output mixing: selections 0 and 1 (the
-// hop terms) at A01, in LDS or global memory, selection 2 (the site term) at
-// A2 in global memory; winv2 their inverse scale; mix the term fields.
+// hop terms) at A01, selection 2 (the site term) at A2, each in LDS or global
+// memory (A2: the packed operands from selection 2 on); winv2 their inverse
+// scale; mix the term fields.
 struct MixOut {
   const uint4 *A01, *A2;
   const float *winv2;
@@ -252,8 +253,8 @@ QCU_D void fused_epilogue(const MixOut &mo, const f16v (&acc)[3][2], float oscal
         ah = as_h8(mo.A01[((k * 4 + ks) * 2) * 64 + lane]);
         al = as_h8(mo.A01[((k * 4 + ks) * 2 + 1) * 64 + lane]);
       } else {
-        ah = as_h8(mo.A2[((8 + ks) * 2) * 64 + lane]);
-        al = as_h8(mo.A2[((8 + ks) * 2 + 1) * 64 + lane]);
+        ah = as_h8(mo.A2[(ks * 2) * 64 + lane]);
+        al = as_h8(mo.A2[(ks * 2 + 1) * 64 + lane]);
       }
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
@@ -446,7 +447,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
   // once the accumulators are split in the fused epilogue
   auto finish = [&](const Chunk &k, auto pre) {
     if constexpr (FUSE) {
-      const MixOut mo{lw + NA, a.A2, a.winv2, a.mix};
+      const MixOut mo{lw + NA, a.A2 + 2 * 4 * 2 * 64, a.winv2, a.mix};
       fused_epilogue(mo, acc, oscale, k.i, k.p, k.ok, gq, lane, field, plane, pre);
     } else {
       pre();
@@ -910,17 +911,20 @@ __global__ void __launch_bounds__(256) k_net_pack_second(const float2 *W1, const
 }
 
 // MIX: the epilogue also writes the next layer's single-output term fields
-// (fused_epilogue with operands in global memory)
+// (fused_epilogue with the hop terms' operands in global memory, the site
+// term's in LDS)
 template <int WAVES, bool MIX>
 __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
   typedef SplitShape<32> SH;
   constexpr int NH = 8 * 2 * 4 * 2 * 64;  // the second layer's hop-term operands (split packing, terms 0..7)
-  __shared__ uint4 lw[NH + kFirstPackOps];
+  constexpr int N2 = MIX ? 4 * 2 * 64 : 0;  // the mixing's site-term operands (8 KiB: 156 KiB in all)
+  __shared__ uint4 lw[NH + kFirstPackOps + N2];
   if (a.st && a.st->done) return;  // uniform over the grid
   for (int k = threadIdx.x; k < NH; k += blockDim.x) lw[k] = a.Ah[k];
   for (int k = threadIdx.x; k < kFirstPackOps; k += blockDim.x) lw[NH + k] = a.A[k];
+  for (int k = threadIdx.x; k < N2; k += blockDim.x) lw[NH + kFirstPackOps + k] = a.A2[2 * 4 * 2 * 64 + k];
   __syncthreads();
-  const uint4 *lh = lw, *le = lw + NH;
+  const uint4 *lh = lw, *le = lw + NH, *ls2 = lw + NH + kFirstPackOps;
   const Geom g = a.g;
   // operand scales: s0 the input's, s1 the first layer's even sites'
   const float s0 = pow2_scale(__uint_as_float(*a.in_max), 14);
@@ -1060,7 +1064,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
     if constexpr (MIX) {
       // the hop-term fields of the single-output mixing; the odd pass's site
       // operand is loaded once the accumulators are split
-      const MixOut mo{a.A2, a.A2, a.winv2, a.mix};
+      const MixOut mo{a.A2, ls2, a.winv2, a.mix};
       fused_epilogue(mo, acc, oe, ic, 0, okc, gq, lane, field, plane, [&]() { load_in(vb, ic); });
     }
     // odd output sites: the site term S on lane group 0 first, then the
@@ -1114,7 +1118,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
     if constexpr (MIX) {
       if (okc) store(ic, 1, oo);
       // the site-term field; the next chunk's first loads once the accumulators are split
-      const MixOut mo{a.A2, a.A2, a.winv2, a.mix};
+      const MixOut mo{a.A2, ls2, a.winv2, a.mix};
       fused_epilogue(mo, acc, oo, ic, 1, okc, gq, lane, field, plane, [&]() {
         if (c + cstep < nchunk) begin(c + cstep);
       });

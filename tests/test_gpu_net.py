@@ -349,19 +349,26 @@ def test_split_layers_follow_the_input_scale(orc, scale):
 # sites and net_layer_second, which forms the second layer's even outputs from
 # the input with combined matrices W2_t W1_8 + W2_8 W1_t and takes its odd
 # outputs' site term from the input too, rescaling its accumulators between
-# the input's scale and the first layer's.  Scaling W1 by a and W2 by 1/a
-# leaves the network unchanged while moving the first layer's output far
-# from the input's size; the input itself is also scaled.
-@pytest.mark.parametrize("a,scale", [(1.0, 1.0), (1e-12, 1.0), (1e12, 1.0), (1.0, 1e-20), (1e-6, 1e15)])
-def test_second_layer_scales(orc, a, scale):
-    """apply_odd of a 1-8-8-8-1 network with W1 * a, W2 / a and input * scale
-    against the oracle's K(x) (FP32 bound 2e-5)"""
+# the input's scale and the first layer's.  The third layer takes its site
+# term at its even output sites from the input as well (W3_8 times those
+# combined matrices), rescaling between the input's scale and the second
+# layer's.  Scaling W1 by a and W2 by 1/a leaves the network unchanged while
+# moving the first layer's output far from the input's size; W2 by b and W3
+# by 1/b does the same for the second layer's output; the input itself is
+# also scaled.
+@pytest.mark.parametrize("a,b,scale", [(1.0, 1.0, 1.0), (1e-12, 1.0, 1.0), (1e12, 1.0, 1.0), (1.0, 1.0, 1e-20),
+                                       (1e-6, 1.0, 1e15), (1.0, 1e-12, 1.0), (1.0, 1e12, 1e-10),
+                                       (1e6, 1e-9, 1e12)])
+def test_second_layer_scales(orc, a, b, scale):
+    """apply_odd of a 1-8-8-8-1 network with W1 * a, W2 * b / a, W3 / b and
+    input * scale against the oracle's K(x) (FP32 bound 2e-5)"""
     L, chan, tb = (8, 4, 4, 8), (1, 8, 8, 8, 1), -1
     U, lat, net, W = _setup(orc, L, chan, tb, 220)
     n0, n1 = 9 * 1 * 8 * 16, 9 * 8 * 8 * 16
     Ws = W.copy()
     Ws[:n0] *= a
-    Ws[n0:n0 + n1] /= a
+    Ws[n0:n0 + n1] *= b / a
+    Ws[n0 + n1:n0 + 2 * n1] /= b
     net.set_weights(Ws)
     xo = orc.gauss((orc.vol(L) // 2, 4, 3), 221)
     full = np.zeros((1, 2) + xo.shape, complex)
@@ -369,7 +376,7 @@ def test_second_layer_scales(orc, a, scale):
     ref, _ = orc.net_forward(full, W, chan, U, L, tb)
     got = host(net.apply_odd(dev(xo * scale))).ravel() / scale
     e = rel(got, ref.reshape(2, -1)[1])
-    log(f"W1 x {a:g}, W2 / {a:g}, input x {scale:g}: apply_odd vs oracle {e:.1e}")
+    log(f"W1 x {a:g}, W2 x {b:g} / {a:g}, W3 / {b:g}, input x {scale:g}: apply_odd vs oracle {e:.1e}")
     assert e < 2e-5
     net.close()
 
