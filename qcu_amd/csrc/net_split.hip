@@ -280,7 +280,7 @@ QCU_D void fused_epilogue(const MixOut &mo, const f16v (&acc)[3][2], float oscal
         v.y = dy[l0 % 3][e0 + 1] * yscale;
         v.z = dy[l1 % 3][e1] * yscale;
         v.w = dy[l1 % 3][e1 + 1] * yscale;
-        st_at<float4>(ob, (size_t)(3 * g + q) * plane, off, v);
+        st_nt(ob, (size_t)(3 * g + q) * plane, off, v);
       }
     }
   }
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_layer_split(NetSplitArgs a) 
             f.z = acc[l1 % 3][mt][e1] * oscale;
             f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
             omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
-            st_at<float4>(ob, (size_t)(3 * (sb / 2) + q) * plane, off, f);
+            st_nt(ob, (size_t)(3 * (sb / 2) + q) * plane, off, f);
           }
         }
     }
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
           f.z = acc[l1 % 3][mt][e1] * oscale;
           f.w = acc[l1 % 3][mt][e1 + 1] * oscale;
           omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
-          st_at<float4>(ob, (size_t)(3 * gq + q) * plane, off, f);
+          st_nt(ob, (size_t)(3 * gq + q) * plane, off, f);
         }
       }
   };
@@ -995,7 +995,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
           f.z = acc[l1 % 3][mt][e1] * os;
           f.w = acc[l1 % 3][mt][e1 + 1] * os;
           omax = fmaxf(omax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
-          st_at<float4>(ob, (size_t)(3 * gq + q) * plane, off, f);
+          st_nt(ob, (size_t)(3 * gq + q) * plane, off, f);
         }
       }
   };

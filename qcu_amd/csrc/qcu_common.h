@@ -107,6 +107,14 @@ template <class V>
 QCU_D void st_at(void *base, size_t comp_bytes, unsigned lane_off, const V &v) {
   *(V *)((char *)base + comp_bytes + lane_off) = v;
 }
+// the same as a non-temporal store: for fields written once and read by a
+// later kernel (the network's activations and term fields), which then need
+// not displace the lines that the writing kernel still reads
+QCU_D void st_nt(void *base, size_t comp_bytes, unsigned lane_off, const float4 &v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, (f4 *)((char *)base + comp_bytes + lane_off));
+}
 
 // ---------------------------------------------------------------------------
 // precision tags
