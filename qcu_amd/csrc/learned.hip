@@ -87,11 +87,16 @@ struct NetLayerArgs {
 };
 
 // the colour matrix of term tg at this site (FP32 links)
+// NT: non-temporal loads (links the kernel reads once)
+template <bool NT = false>
 QCU_D void term_link_matrix(const LinkPtr &U, const TermGeo &tg, cf (&m)[9]) {
   float4 raw[3];
   if (tg.hop) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) raw[k] = ld_at<float4>(U.a, ((size_t)tg.slab * 3 + k) * U.vh * 16, (unsigned)tg.li * 16u);
+    for (int k = 0; k < 3; ++k) {
+      const size_t cb = ((size_t)tg.slab * 3 + k) * U.vh * 16;
+      raw[k] = NT ? ld_nt(U.a, cb, (unsigned)tg.li * 16u) : ld_at<float4>(U.a, cb, (unsigned)tg.li * 16u);
+    }
   }
   term_matrix(raw, tg.hop, tg.fwd, tg.ph, m);
 }
@@ -578,7 +583,8 @@ struct NetGatherArgs {
 // DIRECT: the odd output goes straight into a vector of precision P (the
 // solver's), instead of channel 0 of the FP32 output block.  TWO: the hop
 // terms are the sums of two sets of term fields, the site term is the
-// second set's.
+// second set's.  Every term field and, with one output parity, every link is
+// read once: non-temporal loads.
 template <int P = P32, bool DIRECT = false, bool TWO = false>
 __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
   if (a.st && a.st->done) return;
@@ -587,18 +593,27 @@ __global__ void __launch_bounds__(256) k_net_gather(NetGatherArgs a) {
   const int p = a.parity0 + blockIdx.y;
   if (i >= g.vh) return;
   const Site site = decode(g, i, p);
+  auto load = [&](const SpinorPtr &f, int j, cf(&v)[12]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float4 q = ld_nt(f.p, (size_t)k * f.stride * 16, (unsigned)j * 16u);
+      v[2 * k] = mk(q.x, q.y);
+      v[2 * k + 1] = mk(q.z, q.w);
+    }
+  };
   cf acc[12];
-  SpinorIO<P32>::load(chan_field(TWO ? a.mix2 : a.mix, 8, p, g.vh), i, acc);
+  load(chan_field(TWO ? a.mix2 : a.mix, 8, p, g.vh), i, acc);
 #pragma unroll 2
   for (int t = 0; t < 8; ++t) {
     const TermGeo tg = term_geo(g, site, i, p, t);
     cf m[9];
-    term_link_matrix(a.U, tg, m);
+    if (gridDim.y == 1) term_link_matrix<true>(a.U, tg, m);
+    else term_link_matrix(a.U, tg, m);
     cf v[12];
-    SpinorIO<P32>::load(chan_field(a.mix, t, tg.jp, g.vh), tg.j, v);
+    load(chan_field(a.mix, t, tg.jp, g.vh), tg.j, v);
     if constexpr (TWO) {
       cf w[12];
-      SpinorIO<P32>::load(chan_field(a.mix2, t, tg.jp, g.vh), tg.j, w);
+      load(chan_field(a.mix2, t, tg.jp, g.vh), tg.j, w);
 #pragma unroll
       for (int k = 0; k < 12; ++k) v[k] = v[k] + w[k];
     }

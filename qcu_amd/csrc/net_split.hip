@@ -719,10 +719,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) q[k] = ld_at<float4>(in1, k * plane, (unsigned)j * 16u);
   };
+  // each link is read once (by the even sites' hops)
   auto load_link = [&](const TermGeo &tg, float4 (&l)[3]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      l[k] = ld_at<float4>(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
+    for (int k = 0; k < 3; ++k) l[k] = ld_nt(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
   };
   // chunk c: the lane's site (lanes past the last site compute a copy of a
   // valid site and store nothing) and the first hop pair's loads

@@ -115,6 +115,13 @@ QCU_D void st_nt(void *base, size_t comp_bytes, unsigned lane_off, const float4 
   const f4 w = {v.x, v.y, v.z, v.w};
   __builtin_nontemporal_store(w, (f4 *)((char *)base + comp_bytes + lane_off));
 }
+// a non-temporal load: for data the kernel reads once, which then need not
+// displace the lines it reads again
+QCU_D float4 ld_nt(const void *base, size_t comp_bytes, unsigned lane_off) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 w = __builtin_nontemporal_load((const f4 *)((const char *)base + comp_bytes + lane_off));
+  return make_float4(w.x, w.y, w.z, w.w);
+}
 
 // ---------------------------------------------------------------------------
 // precision tags
