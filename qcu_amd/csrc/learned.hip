@@ -1366,7 +1366,8 @@ struct Net : Precond {
   // split-FP16 hidden layers (8 -> 4 or 8 channels; QCU_NET_SPLIT=0 keeps them
   // on the FP32 matrix-core kernel)
   bool use_split = true;
-  int split_order = 0;          // chunk order of the split kernel (QCU_NET_SPLIT_ORDER)
+  int split_order = -1;         // chunk order of the split kernel (QCU_NET_SPLIT_ORDER; -1: z slabs per XCD
+                                // for one output parity, else interleaved)
   int split_ns = 32;            // sites per wave of the split kernel (QCU_NET_SPLIT_SHAPE)
   bool split_wgrad = true;      // 8 -> 8 weight gradients on split FP16 (QCU_NET_SPLIT_WGRAD=0: FP32)
   bool wg_lds = true;           // weight-gradient operands staged through LDS (QCU_NET_WGRAD_LDS=0: direct loads)
@@ -1597,7 +1598,7 @@ int Net::layer(SpinorPtr in, SpinorPtr out, const float2 *Wl, int cin, int cout,
     b.parity0 = parity0;
     b.npar = nparity;
     b.ns = split_ns;
-    b.order = split_order;
+    b.order = split_order >= 0 ? split_order : (nparity == 1 ? 2 : 0);
     b.st = st;
     b.A2 = (const uint4 *)io->A2;
     b.winv2 = io->winv2;

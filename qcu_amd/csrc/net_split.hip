@@ -1021,10 +1021,23 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
   Site se{};
   TermGeo tg{};
   float4 lraw[3], vb[6];
+  // Chunk order: where the lattice splits into whole z slabs, one per XCD
+  // (the grid a multiple of 8 blocks, block b on XCD b mod 8), each XCD sweeps
+  // its slab plane by plane in t, so that the rows its chunks read through
+  // the z and t hops are read by its own waves while they are still in its L2;
+  // otherwise chunks are interleaved over the grid.
+  const int cpp = (g.lxh * g.ly) / 32;  // chunks per z plane
+  const bool slab = (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && cpp * 32 == g.lxh * g.ly;
+  const int dz = g.lz >> 3;
+  auto chunk_site = [&](int c) {
+    if (!slab) return c;
+    const int within = c % cpp, q = c / cpp;
+    return ((q / dz) * g.lz + (int)(blockIdx.x & 7) * dz + q % dz) * cpp + within;
+  };
   // chunk c: the lane's site (lanes past the last site compute a copy of a
   // valid site and store nothing) and the even pass's first loads
   auto begin = [&](int c) {
-    const int base = c * 32;
+    const int base = chunk_site(c) * 32;
     i = base + n;
     ok = i < g.vh;
     if (!ok) i = base;
@@ -1033,11 +1046,12 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
     load_link(tg, lraw);
     load_in(vb, tg.j);
   };
-  const int cstep = gridDim.x * WAVES;
-  int c = blockIdx.x * WAVES + wave;
-  if (c < nchunk) begin(c);
+  const int cstep = slab ? (int)(gridDim.x >> 3) * WAVES : gridDim.x * WAVES;
+  const int cend = slab ? cpp * dz * g.lt : nchunk;
+  int c = slab ? (int)(blockIdx.x >> 3) * WAVES + wave : blockIdx.x * WAVES + wave;
+  if (c < cend) begin(c);
 #pragma unroll 1
-  for (; c < nchunk; c += cstep) {
+  for (; c < cend; c += cstep) {
     const int ic = i;
     const bool okc = ok;
     // even output sites: K step ks = the hops in direction ks of the input
@@ -1120,10 +1134,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
       // the site-term field; the next chunk's first loads once the accumulators are split
       const MixOut mo{a.A2, ls2, a.winv2, a.mix};
       fused_epilogue(mo, acc, oo, ic, 1, okc, gq, lane, field, plane, [&]() {
-        if (c + cstep < nchunk) begin(c + cstep);
+        if (c + cstep < cend) begin(c + cstep);
       });
     } else {
-      if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads, ahead of the stores
+      if (c + cstep < cend) begin(c + cstep);  // the next chunk's first loads, ahead of the stores
       if (okc) store(ic, 1, oo);
     }
   }

@@ -301,8 +301,10 @@ def test_split_shapes_and_orders(orc, monkeypatch, shape, order):
 # The preconditioner application folds the last layer (8 -> 1) into the
 # epilogue of the 8 -> 8 layer before it: that kernel writes the last layer's
 # mixed term fields instead of its own output channels.
+# (16, 8, 16, 16) is large enough for the z-slab chunk order (a grid of a
+# multiple of 8 blocks, Lz a multiple of 8) of the second and third layers.
 @pytest.mark.parametrize("L,chan", [((8, 4, 4, 8), (1, 8, 8, 8, 1)), ((6, 2, 2, 2), (1, 8, 8, 1)),
-                                    ((10, 6, 4, 4), (1, 8, 8, 8, 1))])
+                                    ((10, 6, 4, 4), (1, 8, 8, 8, 1)), ((16, 8, 16, 16), (1, 8, 8, 8, 1))])
 def test_fused_last_layer_apply_odd(orc, monkeypatch, L, chan):
     """apply_odd with the fused last layer against the oracle (FP32 bound 2e-5)
     and against the unfused FP32 matrix-core path (2e-6)"""
