@@ -636,6 +636,40 @@ __global__ void __launch_bounds__(256) k_net_pack_first(const float2 *W, uint4 *
   }
 }
 
+// Chunk order of the kernels that take 32 sites of one parity per chunk
+// (k_net_first, k_net_second): where the lattice splits into whole z slabs,
+// one per XCD (the grid a multiple of 8 blocks, block b on XCD b mod 8), each
+// XCD sweeps its slab plane by plane in t, so that the rows its chunks read
+// through the z and t hops are read by its own waves while they are still in
+// its L2; otherwise chunks are interleaved over the grid.
+struct SlabOrder {
+  bool slab;
+  int cpp, dz;           // chunks per z plane, z planes per slab
+  int c0, cend, cstep;   // the wave's chunks: c0, c0 + cstep, ... < cend
+};
+QCU_D SlabOrder slab_order(const Geom &g, int waves, int wave) {
+  SlabOrder o;
+  o.cpp = (g.lxh * g.ly) / 32;
+  o.dz = g.lz >> 3;
+  o.slab = (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && o.cpp * 32 == g.lxh * g.ly;
+  if (o.slab) {
+    o.c0 = (int)(blockIdx.x >> 3) * waves + wave;
+    o.cend = o.cpp * o.dz * g.lt;
+    o.cstep = (int)(gridDim.x >> 3) * waves;
+  } else {
+    o.c0 = blockIdx.x * waves + wave;
+    o.cend = (g.vh + 31) / 32;
+    o.cstep = gridDim.x * waves;
+  }
+  return o;
+}
+// chunk c's index in site order (its first site is 32 times it)
+QCU_D int slab_chunk(const SlabOrder &o, const Geom &g, int c) {
+  if (!o.slab) return c;
+  const int within = c % o.cpp, q = c / o.cpp;
+  return ((q / o.dz) * g.lz + (int)(blockIdx.x & 7) * o.dz + q % o.dz) * o.cpp + within;
+}
+
 // EVEN_ONLY: the even output sites only (the odd ones are then never read:
 // the next layer takes their site term from the input itself, k_net_second)
 template <int WAVES, bool EVEN_ONLY>
@@ -650,7 +684,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
   const float oscale = a.winv[0] / sin_;
   const int lane = threadIdx.x & 63, gq = lane >> 5, n = lane & 31;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nchunk = (g.vh + 31) / 32;
   const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
   const char *in1 = a.in + field;  // the input's odd sites
   float omax = 0.f;
@@ -724,15 +757,16 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) l[k] = ld_nt(a.U.a, ((size_t)tg.slab * 3 + k) * plane, (unsigned)tg.li * 16u);
   };
-  // chunk c: the lane's site (lanes past the last site compute a copy of a
-  // valid site and store nothing) and the first hop pair's loads
   int i = 0;
   bool ok = false;
   Site se{};
   TermGeo tg{};
   float4 lraw[3], vb[6];
+  const SlabOrder sl = slab_order(g, WAVES, wave);
+  // chunk c: the lane's site (lanes past the last site compute a copy of a
+  // valid site and store nothing) and the first hop pair's loads
   auto begin = [&](int c) {
-    const int base = c * 32;
+    const int base = slab_chunk(sl, g, c) * 32;
     i = base + n;
     ok = i < g.vh;
     if (!ok) i = base;
@@ -741,11 +775,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
     load_link(tg, lraw);
     load_v(vb, tg.j);
   };
-  const int cstep = gridDim.x * WAVES;
-  int c = blockIdx.x * WAVES + wave;
-  if (c < nchunk) begin(c);
+  const int cstep = sl.cstep, cend = sl.cend;
+  int c = sl.c0;
+  if (c < cend) begin(c);
 #pragma unroll 1
-  for (; c < nchunk; c += cstep) {
+  for (; c < cend; c += cstep) {
     // even output sites: K step ks = the hops in direction ks, forward on
     // lane group 0, backward on group 1; the next step's link and
     // neighbour are loaded while this step's products run
@@ -764,7 +798,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
         load_link(tg, lraw);
         load_v(vb, tg.j);
       } else if constexpr (EVEN_ONLY) {
-        if (c + cstep < nchunk) begin(c + cstep);  // the next chunk's first loads
+        if (c + cstep < cend) begin(c + cstep);  // the next chunk's first loads
       } else {
         load_v(vb, i);  // the odd pass: the site's own input
       }
@@ -782,7 +816,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_first(NetFirstArgs a) {
       const int io = i;
       const bool oko = ok;
       __builtin_amdgcn_sched_barrier(0);
-      if (c + cstep < nchunk) begin(c + cstep);
+      if (c + cstep < cend) begin(c + cstep);
       __builtin_amdgcn_sched_barrier(0);
       const float f = gq == 0 ? sin_ : 0.f;
 #pragma unroll
@@ -934,7 +968,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
   const float rs = (a.winv[1] * s1) / (a.winvh[0] * s0);  // the site term's part -> the hop terms' scale
   const int lane = threadIdx.x & 63, gq = lane >> 5, n = lane & 31;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nchunk = (g.vh + 31) / 32;
   const size_t field = (size_t)g.vh * 96, plane = (size_t)g.vh * 16;
   const unsigned chan_off = (unsigned)(2 * field);  // one channel (both parities)
   const char *in1 = a.in + field;  // the input's odd sites
@@ -1021,23 +1054,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
   Site se{};
   TermGeo tg{};
   float4 lraw[3], vb[6];
-  // Chunk order: where the lattice splits into whole z slabs, one per XCD
-  // (the grid a multiple of 8 blocks, block b on XCD b mod 8), each XCD sweeps
-  // its slab plane by plane in t, so that the rows its chunks read through
-  // the z and t hops are read by its own waves while they are still in its L2;
-  // otherwise chunks are interleaved over the grid.
-  const int cpp = (g.lxh * g.ly) / 32;  // chunks per z plane
-  const bool slab = (gridDim.x & 7) == 0 && (g.lz & 7) == 0 && cpp * 32 == g.lxh * g.ly;
-  const int dz = g.lz >> 3;
-  auto chunk_site = [&](int c) {
-    if (!slab) return c;
-    const int within = c % cpp, q = c / cpp;
-    return ((q / dz) * g.lz + (int)(blockIdx.x & 7) * dz + q % dz) * cpp + within;
-  };
+  const SlabOrder sl = slab_order(g, WAVES, wave);
   // chunk c: the lane's site (lanes past the last site compute a copy of a
   // valid site and store nothing) and the even pass's first loads
   auto begin = [&](int c) {
-    const int base = chunk_site(c) * 32;
+    const int base = slab_chunk(sl, g, c) * 32;
     i = base + n;
     ok = i < g.vh;
     if (!ok) i = base;
@@ -1046,9 +1067,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_net_second(NetSecondArgs a) {
     load_link(tg, lraw);
     load_in(vb, tg.j);
   };
-  const int cstep = slab ? (int)(gridDim.x >> 3) * WAVES : gridDim.x * WAVES;
-  const int cend = slab ? cpp * dz * g.lt : nchunk;
-  int c = slab ? (int)(blockIdx.x >> 3) * WAVES + wave : blockIdx.x * WAVES + wave;
+  const int cstep = sl.cstep, cend = sl.cend;
+  int c = sl.c0;
   if (c < cend) begin(c);
 #pragma unroll 1
   for (; c < cend; c += cstep) {
