@@ -1205,7 +1205,7 @@ int net_layer_second(const NetSecondArgs &a, hipStream_t s) {
     QCU_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (ncu <= 0) ncu = 256;
   }
-  // 8 waves per block, one block per CU (the operands fill 148 KiB of its LDS)
+  // 8 waves per block, one block per CU (the operands fill 148 KiB of its LDS, 156 KiB with the mixing)
   constexpr int waves = 8;
   const int nchunk = (a.g.vh + 31) / 32;
   int nblk = ncu;
