@@ -1172,10 +1172,11 @@ __global__ void k_net_transpose(const float2 *W, float2 *Wt, int cin, int cout) 
 
 // embed an odd-site vector (precision P) as channel 0 of a full FP32 field
 // (rec: receives the largest |component|, or null; then 4 sites per thread
-// and one atomic per block of 256 threads)
+// and one atomic per block of 256 threads); zero_even: the even sites are
+// written as zeros (else left as they are, for passes that never read them)
 template <int P>
 __global__ void __launch_bounds__(256) k_net_embed(SpinorPtr full, SpinorPtr odd, int vh, const SysState *st,
-                                                  unsigned *rec = nullptr) {
+                                                  unsigned *rec = nullptr, int zero_even = 1) {
   if (st && st->done) return;
   typedef typename real_of<P>::type T;
   const int per = rec ? 4 : 1;
@@ -1193,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_net_embed(SpinorPtr full, SpinorPtr odd
       mx = fmaxf(mx, fmaxf(fabsf(w[c].re), fabsf(w[c].im)));
     }
     SpinorIO<P32>::store(chan_field(full, 0, 1, vh), i, w);
-    SpinorIO<P32>::store(chan_field(full, 0, 0, vh), i, z);
+    if (zero_even) SpinorIO<P32>::store(chan_field(full, 0, 0, vh), i, z);
   }
   if (!rec) return;  // uniform
   __shared__ float red[256];
@@ -1416,7 +1417,10 @@ struct Net : Precond {
   bool input_fresh = false;  // load_input ran since the last forward pass: maxima cleared, input's recorded
   // act[0] <- the odd vector in (precision prec) as channel 0, zero on the
   // even sites; clears the field maxima of the next forward pass first
-  int load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s);
+  int load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s, bool zero_even = true);
+  // the preconditioner's forward pass (apply) reads the input's odd sites only:
+  // its first two layers run as net_layer_second
+  bool apply_reads_odd_input_only() const { return second_split() && !(fusable() && nl == 3); }
   // the last layer (8 -> 1 channels) can be folded into the epilogue of the
   // 8 -> 8 layer before it when only the network output is needed
   bool fusable() const {
@@ -1948,7 +1952,7 @@ int Net::backward(SpinorPtr gout, hipStream_t s, bool want_grad_in, SpinorPtr gr
   return 0;
 }
 
-int Net::load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s) {
+int Net::load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s, bool zero_even) {
   const int vh = lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
   const SpinorPtr a0 = act_ptr(0);
@@ -1957,10 +1961,11 @@ int Net::load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s) {
   if (any) QCU_TRY(reset_max(0, s));
   unsigned *rec = first_split() ? (unsigned *)split_io(0, 0).in_max : nullptr;
   const unsigned nbr = rec ? (unsigned)ndiv(vh, 4 * bs) : nb;
+  const int ze = zero_even ? 1 : 0;
   switch (prec) {
-    case P64: hipLaunchKernelGGL(k_net_embed<P64>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
-    case P32: hipLaunchKernelGGL(k_net_embed<P32>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
-    default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec); break;
+    case P64: hipLaunchKernelGGL(k_net_embed<P64>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec, ze); break;
+    case P32: hipLaunchKernelGGL(k_net_embed<P32>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec, ze); break;
+    default: hipLaunchKernelGGL(k_net_embed<P16>, dim3(nbr), dim3(bs), 0, s, a0, in, vh, st, rec, ze); break;
   }
   QCU_HIP(hipGetLastError());
   input_fresh = any;
@@ -1970,7 +1975,7 @@ int Net::load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s) {
 int Net::apply(Solver &S, SpinorPtr out, SpinorPtr in, int prec, SysState *st) {
   const int vh = lat->g.vh, bs = 256;
   const unsigned nb = (unsigned)ndiv(vh, bs);
-  QCU_TRY(load_input(in, prec, st, S.stream));
+  QCU_TRY(load_input(in, prec, st, S.stream, !apply_reads_odd_input_only()));
   bool wrote = false;
   QCU_TRY(forward(st, S.stream, true, 0, false, &out, prec, &wrote));
   if (wrote) return 0;  // the fused last layer wrote out
@@ -2290,7 +2295,7 @@ int qcuNetApplyOdd(QcuNetHandle h, void *out, const void *in) {
   NetTrainer tr{net, s};
   hipLaunchKernelGGL(k_load_odd, dim3(nb), dim3(bs), 0, s, tr.odd(0), (const double2 *)in, vh);
   QCU_HIP(hipGetLastError());
-  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
+  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s, !net.apply_reads_odd_input_only()));
   QCU_TRY(net.forward(nullptr, s, true, 0, false));
   SpinorPtr o1 = net.act_ptr(net.nl);
   o1.p = (char *)o1.p + (size_t)vh * 96;  // channel 0, parity 1
@@ -2388,7 +2393,7 @@ int qcuNetBenchPrecondLayer(QcuNetHandle h, int layer, int iters, double *ms) {
   const size_t nf = (size_t)vh * 24;
   hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)ndiv(nf, 256)), dim3(256), 0, s, (float *)tr.odd(0).p, nf);
   QCU_HIP(hipGetLastError());
-  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
+  QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s, !net.apply_reads_odd_input_only()));
   bool wrote = false;
   const SpinorPtr o1 = tr.odd(1);
   QCU_TRY(net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote));
@@ -2427,7 +2432,7 @@ int qcuNetBenchApply(QcuNetHandle h, int iters, double *ms) {
   NetTrainer tr{net, s};
   // as in the solver: the fused last layer writes the odd output vector itself
   auto once = [&]() {
-    QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s));
+    QCU_TRY(net.load_input(tr.odd(0), P32, nullptr, s, !net.apply_reads_odd_input_only()));
     bool wrote = false;
     const SpinorPtr o1 = tr.odd(1);
     QCU_TRY(net.forward(nullptr, s, true, 0, false, &o1, P32, &wrote));
