@@ -1416,7 +1416,9 @@ struct Net : Precond {
   float *fpk2winv() const { return (float *)((char *)fpk2.p + kFusePackBytes); }
   bool input_fresh = false;  // load_input ran since the last forward pass: maxima cleared, input's recorded
   // act[0] <- the odd vector in (precision prec) as channel 0, zero on the
-  // even sites; clears the field maxima of the next forward pass first
+  // even sites (zero_even false: the even sites are left unwritten, for a
+  // forward pass that never reads them); clears the field maxima of the next
+  // forward pass first
   int load_input(SpinorPtr in, int prec, const SysState *st, hipStream_t s, bool zero_even = true);
   // the preconditioner's forward pass (apply) reads the input's odd sites only:
   // its first two layers run as net_layer_second
