@@ -101,9 +101,10 @@ def _schur_ops(orc, U, L, kappa, csw, tb):
 # In training the network input and the output gradient are zero on the even
 # sites, and the first and last layers' weight gradients sweep only the
 # parities that can contribute; (6, 2, 2, 2) (24 sites per parity) ends those
-# sweeps on a partial 4-site group and a partial 16-site wave.
+# sweeps on a partial 4-site group and a partial 16-site wave; (16, 8, 16, 16)
+# runs the first layer's kernel in its z-slab chunk order.
 @pytest.mark.parametrize("L,chan,csw", [((8, 4, 4, 8), (1, 4, 4, 1), 0.0), ((8, 4, 4, 8), (1, 4, 4, 1), 1.0),
-                                        ((6, 2, 2, 2), (1, 8, 3, 1), 1.0)])
+                                        ((6, 2, 2, 2), (1, 8, 3, 1), 1.0), ((16, 8, 16, 16), (1, 8, 8, 8, 1), 1.0)])
 def test_training_gradient_vs_oracle(orc, L, chan, csw):
     tb, kappa = -1, 0.124
     U, lat, net, W = _setup(orc, L, chan, tb, 120, csw=csw, kappa=kappa)
